@@ -1,0 +1,219 @@
+#include "ast.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+namespace combustion::ast {
+
+const char* op_name(Op op) {
+  static const char* names[] = {"const", "value", "neg", "min", "max", "add", "sub", "le", "mul", "div", "sqrt", "exp", "ln", "sq"};
+  return names[(int)op];
+}
+
+Block::Block(std::vector<std::string> input_names) {
+  f_.n_input = (int32_t)input_names.size();
+  f_.values = std::move(input_names);
+}
+
+Function Block::finish(std::vector<E> outputs) {
+  f_.output = std::move(outputs);
+  return std::move(f_);
+}
+
+E Block::make(Op op, E a, E b, double c) {
+  uint64_t bits;
+  std::memcpy(&bits, &c, 8);
+  uint64_t h = (uint64_t)op * 0x9E3779B97F4A7C15ull;
+  h ^= (uint64_t)(uint32_t)a + 0x7F4A7C159E3779B9ull + (h << 6) + (h >> 2);
+  h ^= (uint64_t)(uint32_t)b + 0x94D049BB133111EBull + (h << 6) + (h >> 2);
+  h ^= bits + 0xBF58476D1CE4E5B9ull + (h << 6) + (h >> 2);
+  auto& bucket = intern_[h];
+  for (E e : bucket) {
+    const Node& n = f_.nodes[e];
+    uint64_t nb;
+    std::memcpy(&nb, &n.c, 8);
+    if (n.op == op && n.a == a && n.b == b && nb == bits) return e;
+  }
+  f_.nodes.push_back(Node{op, a, b, c});
+  E e = (E)f_.nodes.size() - 1;
+  bucket.push_back(e);
+  return e;
+}
+
+// ast/lib.rs:155 f64(): finite, and finite once demoted to f32.
+E Block::c(double x) {
+  if (!std::isfinite(x)) throw Error("non-finite constant");
+  if (!std::isfinite((float)x)) throw Error("constant " + std::to_string(x) + " overflows f32");
+  return make(Op::Const, -1, -1, x);
+}
+E Block::v(int32_t id) { return make(Op::Value, id, -1); }
+
+std::optional<double> Block::f64(E e) const {
+  if (f_.nodes[e].op == Op::Const) return f_.nodes[e].c;
+  return std::nullopt;
+}
+bool Block::is_leaf(E e) const { return f_.nodes[e].op == Op::Const || f_.nodes[e].op == Op::Value; }
+// Expr::f32() comparisons of ast/lib.rs:198-229 test the constant after demotion to f32.
+bool Block::is_f32(E e, float x) const { return f_.nodes[e].op == Op::Const && (float)f_.nodes[e].c == x; }
+
+E Block::neg(E x) { return make(Op::Neg, x, -1); }
+
+E Block::add(E a, E b) {
+  if (f64(a) && f64(b)) return c(*f64(a) + *f64(b));
+  if (is_f32(a, 0.f)) return b;
+  if (is_f32(b, 0.f)) return a;
+  return make(Op::Add, a, b);
+}
+
+E Block::sub(E a, E b) {
+  if (f64(a) && f64(b)) return c(*f64(a) - *f64(b));
+  if (is_f32(a, 0.f)) return neg(b);
+  if (is_f32(b, 0.f)) return a;
+  return make(Op::Sub, a, b);
+}
+
+E Block::mul(E a, E b) {
+  if (f64(a) && f64(b)) return c(*f64(a) * *f64(b));
+  if (is_f32(a, 0.f) || is_f32(b, 0.f)) return c(0.);
+  if (is_f32(a, 1.f)) return b;
+  if (is_f32(a, -1.f)) return neg(b);
+  if (is_f32(b, 1.f)) return a;
+  if (is_f32(b, -1.f)) return neg(a);
+  return make(Op::Mul, a, b);
+}
+
+// ast/lib.rs:225-232: division by a constant becomes multiplication by its reciprocal.
+E Block::div(E a, E b) {
+  if (f64(b)) return mul(a, c(1. / *f64(b)));
+  if (is_f32(a, 0.f)) throw Error("division of constant zero");
+  return make(Op::Div, a, b);
+}
+
+E Block::min(E a, E b) { return make(Op::Min, a, b); }
+E Block::max(E a, E b) { return make(Op::Max, a, b); }
+E Block::le(E a, E b) { return make(Op::LessOrEqual, a, b); }
+E Block::sqrt(E x) { return make(Op::Sqrt, x, -1); }
+E Block::sq(E x) { return make(Op::Sq, x, -1); }
+E Block::exp(E x) {
+  if (f64(x)) throw Error("exp of a constant");  // ast/lib.rs:375
+  return make(Op::Exp, x, -1);
+}
+E Block::ln(E x) {
+  if (f64(x)) return c(std::log(*f64(x)));
+  return make(Op::Ln, x, -1);
+}
+
+std::optional<E> Block::sigma(const std::vector<E>& terms) {
+  std::optional<E> s;
+  for (E t : terms) {
+    if (is_f32(t, 0.f)) continue;
+    s = s ? add(*s, t) : t;
+  }
+  return s;
+}
+E Block::sum(const std::vector<E>& terms) {
+  auto s = sigma(terms);
+  if (!s) throw Error("empty sum");
+  return *s;
+}
+
+int32_t Block::value(const std::string& name) {
+  f_.values.push_back(name);
+  return (int32_t)f_.values.size() - 1;
+}
+
+int32_t Block::def(E e, const std::string& name) {
+  if (is_leaf(e)) throw Error("definition of a leaf: " + name);
+  auto it = defined_.find(e);
+  if (it != defined_.end()) return it->second;
+  int32_t id = value(name);
+  Statement s;
+  s.id = id;
+  s.value = e;
+  f_.statements.push_back(std::move(s));
+  defined_[e] = id;
+  return id;
+}
+
+void Block::select(E condition, std::vector<E> t, std::vector<E> f, std::vector<int32_t> results) {
+  Statement s;
+  s.select = true;
+  s.condition = condition;
+  s.t = std::move(t);
+  s.f = std::move(f);
+  s.results = std::move(results);
+  f_.statements.push_back(std::move(s));
+}
+
+Linear linearize(const Function& f) {
+  Linear l;
+  l.n_input = f.n_input;
+  std::vector<int32_t> value_reg(f.values.size(), -1);
+  for (int32_t i = 0; i < f.n_input; i++) value_reg[i] = i;
+  int32_t next = f.n_input;
+  std::vector<int32_t> node_reg(f.nodes.size(), -1);
+  auto emit = [&](LOp op, int32_t a, int32_t b, int32_t c, double k) {
+    l.code.push_back(Instruction{op, next, a, b, c, k});
+    return next++;
+  };
+  std::function<int32_t(E)> eval = [&](E e) -> int32_t {
+    if (node_reg[e] >= 0) return node_reg[e];
+    const Node& n = f.nodes[e];
+    int32_t r;
+    switch (n.op) {
+      case Op::Const: r = emit(LOp::Const, -1, -1, -1, n.c); break;
+      case Op::Value:
+        if (value_reg[n.a] < 0) throw Error("use of undefined value " + f.values[n.a]);
+        return value_reg[n.a];  // not memoized: Select results are assigned after their first reference is built
+      case Op::Neg: r = emit(LOp::Neg, eval(n.a), -1, -1, 0); break;
+      case Op::Sqrt: r = emit(LOp::Sqrt, eval(n.a), -1, -1, 0); break;
+      case Op::Exp: r = emit(LOp::Exp, eval(n.a), -1, -1, 0); break;
+      case Op::Ln: r = emit(LOp::Ln, eval(n.a), -1, -1, 0); break;
+      case Op::Sq: r = emit(LOp::Sq, eval(n.a), -1, -1, 0); break;
+      default: {
+        int32_t a = eval(n.a);
+        int32_t b = eval(n.b);
+        LOp op = n.op == Op::Min ? LOp::Min : n.op == Op::Max ? LOp::Max : n.op == Op::Add ? LOp::Add : n.op == Op::Sub ? LOp::Sub
+                 : n.op == Op::LessOrEqual ? LOp::LessOrEqual : n.op == Op::Mul ? LOp::Mul : LOp::Div;
+        r = emit(op, a, b, -1, 0);
+      }
+    }
+    node_reg[e] = r;
+    return r;
+  };
+  for (const Statement& s : f.statements) {
+    if (!s.select) {
+      value_reg[s.id] = eval(s.value);
+    } else {
+      int32_t cond = eval(s.condition);
+      for (size_t i = 0; i < s.results.size(); i++) {
+        int32_t t = eval(s.t[i]);
+        int32_t fr = eval(s.f[i]);
+        value_reg[s.results[i]] = emit(LOp::Select, cond, t, fr, 0);
+      }
+    }
+  }
+  for (E e : f.output) l.output.push_back(eval(e));
+  l.n_registers = next;
+  return l;
+}
+
+OpCounts count(const Linear& l) {
+  OpCounts c;
+  for (const Instruction& i : l.code) {
+    switch (i.op) {
+      case LOp::Const: c.constants++; break;
+      case LOp::Neg: c.neg++; break;
+      case LOp::Select: c.select++; break;
+      case LOp::Div: c.div++; break;
+      case LOp::Sqrt: c.sqrt++; break;
+      case LOp::Exp: c.exp++; break;
+      case LOp::Ln: c.ln++; break;
+      default: c.alu++;
+    }
+  }
+  return c;
+}
+
+}  // namespace combustion::ast

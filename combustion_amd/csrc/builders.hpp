@@ -1,0 +1,63 @@
+// Program builders: the per-state rates program (reaction.rs) and the mixture-averaged transport program
+// (transport.rs), plus the per-state SoA program that the GPU boundary evaluates.
+#pragma once
+#include <vector>
+
+#include "ast.hpp"
+#include "mechanism.hpp"
+
+namespace combustion {
+
+// reaction.rs:73
+struct Temperatures {
+  ast::E lnT, T, T2, T3, T4, rcpT, rcpT2;
+};
+
+// reaction.rs:312-327: net production rate of every active species.
+std::vector<ast::E> species_rates(ast::Block& f, const Mechanism& m, const Temperatures& Ts, ast::E concentration,
+                                  const std::vector<ast::E>& concentrations);
+// reaction.rs:356-391: inputs [P/R, R/P, T, V, n_0..n_{K-2}], outputs [T dT/dt / T * T, dV/dt, ω_0..ω_{Ka-1}].
+ast::Function rates(const Mechanism& m);
+// reaction.rs:329-354: inputs [P/R, T, V, n_0..n_{K-2}], outputs the net rate of progress of every reaction.
+ast::Function reactions_rates(const Mechanism& m);
+
+// transport.rs:119-137: polynomial fits in ln(T/T0) of degree D-1.
+struct Polynomials {
+  int D = 4;
+  size_t K = 0;
+  std::vector<double> conductivityIVT;   // [K][D]  λ_k/√T
+  std::vector<double> VviscosityIVVT;    // [K][D]  √(η_k/√T)
+  std::vector<double> diffusivityITVT;   // [K][K][D]  (P·D)_kj / T^{3/2}; entries j<k are fitted, j>k mirror them
+};
+Polynomials fit_polynomials(const Species& species, double T0, int D);
+
+// Single-species physics (transport.rs:57-117), exposed for tests.
+namespace transport {
+double viscosity(const Species& s, size_t k, double T);
+double diffusivity(const Species& s, size_t a, size_t b, double T);
+double conductivity(const Species& s, size_t k, double T);
+double omega22(const Species& s, size_t a, size_t b, double T);
+double omega11(const Species& s, size_t a, size_t b, double T);
+}  // namespace transport
+
+// transport.rs:179-214: inputs [Σx, T/T0, x_0..x_{K-2}], outputs [λ/λ0, μ/η0, ρD_k/η0].
+ast::Function properties(const Species& species, const Polynomials& p, double T0, double viscosity, double conductivity);
+
+// Reference scales for transport (the role of cantera.rs:60-76): T0 = 1000 K, η0 = ρ0·(1 m²/s) and
+// λ0 = cp0·η0/W̄0 for the mechanism's initial-state mixture at T0 and 1 atm.
+struct TransportScales {
+  double T0, viscosity, conductivity;
+};
+TransportScales transport_scales(const Mechanism& m, double T0 = 1000.);
+
+// Per-state program over SoA inputs [T, P, Y_0..Y_{K-1}] (K, Pa, mass fractions). Outputs, in order:
+// ω_0..ω_{Ka-1} [mol/m³/s] when rates is set; λ [W/m/K], μ [Pa s], D_0..D_{K-1} [m²/s] when transport is set;
+// and, when rhs is set, dT/dt [K/s] last.
+// Concentrations follow the reference's amounts/volume convention on a mass basis: n_k = Y_k/W_k and
+// 1/V = (P/R)/T / Σ_j Y_j/W_j, so C_k = max(0, n_k)/V for k < K-1 and the bulk species closes C_tot = P/(R T).
+struct ChemistryOptions {
+  bool rates = true, transport = true, rhs = false;
+};
+ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
+
+}  // namespace combustion

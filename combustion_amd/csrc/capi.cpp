@@ -1,0 +1,360 @@
+// extern "C" boundary (include/combustion_amd.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+
+#include "../../include/combustion_amd.h"
+#include "builders.hpp"
+#include "device.hpp"
+#include "emit.hpp"
+#include "forms.hpp"
+#include "mechanism.hpp"
+#include "yaml.hpp"
+
+using namespace combustion;
+
+struct cb_mechanism {
+  std::shared_ptr<const Mechanism> m;
+};
+
+struct cb_program {
+  std::shared_ptr<const Mechanism> mechanism;
+  ast::Function function;
+  ast::Linear linear;
+  ProgramKind kind = ProgramKind::Other;
+  ChemistryDescription chemistry;  // valid when kind == Chemistry
+};
+
+struct cb_module {
+  std::unique_ptr<Form> form;
+};
+
+namespace {
+
+thread_local std::string last_error;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return CB_OK;
+  } catch (const yaml::Error& e) {
+    last_error = std::string("parse: ") + e.what();
+    return CB_ERR_PARSE;
+  } catch (const Unsupported& e) {
+    last_error = e.what();
+    return CB_ERR_UNSUPPORTED;
+  } catch (const ast::Error& e) {
+    last_error = std::string("build: ") + e.what();
+    return std::string(e.what()).find("not implemented") != std::string::npos ? CB_ERR_UNSUPPORTED : CB_ERR_BUILD;
+  } catch (const CompileError& e) {
+    last_error = e.what();
+    return CB_ERR_COMPILE;
+  } catch (const device::Error& e) {
+    last_error = e.what();
+    return std::string(e.what()).rfind("hiprtc", 0) == 0 ? CB_ERR_COMPILE : CB_ERR_DEVICE;
+  } catch (const ArgumentError& e) {
+    last_error = e.what();
+    return CB_ERR_ARGUMENT;
+  } catch (const std::exception& e) {
+    last_error = e.what();
+    return CB_ERR_BUILD;
+  }
+}
+
+void require(bool ok, const char* what) {
+  if (!ok) throw ArgumentError(what);
+}
+
+int write_text(const std::string& s, char* buffer, size_t capacity, size_t* needed) {
+  if (needed) *needed = s.size() + 1;
+  if (!buffer || capacity < s.size() + 1) {
+    last_error = "buffer too small";
+    return CB_ERR_SIZE;
+  }
+  std::memcpy(buffer, s.c_str(), s.size() + 1);
+  return CB_OK;
+}
+
+std::string num(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x > 0 ? "Infinity" : "-Infinity";
+  char buf[40];
+  std::snprintf(buf, sizeof buf, "%.17g", x);
+  return buf;
+}
+
+std::string mechanism_json(const Mechanism& m) {
+  const Species& s = m.species;
+  std::ostringstream o;
+  o << "{\"active\":" << m.active << ",\"species\":[";
+  for (size_t k = 0; k < s.len(); k++) {
+    if (k) o << ",";
+    o << "{\"name\":\"" << s.names[k] << "\",\"molar_mass\":" << num(s.molar_mass[k]) << ",\"composition\":[";
+    for (int e = 0; e < n_elements; e++) o << (e ? "," : "") << s.composition[k][e];
+    const NASA7& t = s.thermodynamics[k];
+    o << "],\"temperature_split\":" << num(t.temperature_split) << ",\"pieces\":[";
+    for (int p = 0; p < 2; p++) {
+      o << (p ? ",[" : "[");
+      for (int i = 0; i < 7; i++) o << (i ? "," : "") << num(t.pieces[p][i]);
+      o << "]";
+    }
+    o << "],\"geometry\":" << (int)s.geometry[k] << ",\"diameter\":" << num(s.diameter[k]) << ",\"well_depth_J\":" << num(s.well_depth_J[k])
+      << ",\"polarizability\":" << num(s.polarizability[k]) << ",\"permanent_dipole_moment\":" << num(s.permanent_dipole_moment[k])
+      << ",\"rotational_relaxation\":" << num(s.rotational_relaxation[k]) << ",\"internal_degrees_of_freedom\":" << num(s.internal_degrees_of_freedom[k])
+      << "}";
+  }
+  o << "],\"reactions\":[";
+  static const char* models[] = {"elementary", "irreversible", "three-body", "pressure-modification", "troe", "sri"};
+  for (size_t r = 0; r < m.reactions.size(); r++) {
+    const Reaction& x = m.reactions[r];
+    if (r) o << ",";
+    o << "{\"equation\":\"" << x.equation << "\",\"model\":\"" << models[(int)x.model] << "\",\"reactants\":[";
+    for (size_t k = 0; k < x.reactants.size(); k++) o << (k ? "," : "") << (int)x.reactants[k];
+    o << "],\"products\":[";
+    for (size_t k = 0; k < x.products.size(); k++) o << (k ? "," : "") << (int)x.products[k];
+    o << "],\"net\":[";
+    for (size_t k = 0; k < x.net.size(); k++) o << (k ? "," : "") << (int)x.net[k];
+    o << "],\"sum_net\":" << x.sum_net;
+    auto rc = [&](const char* name, const RateConstant& c) {
+      o << ",\"" << name << "\":[" << num(c.preexponential_factor) << "," << num(c.temperature_exponent) << "," << num(c.activation_temperature) << "]";
+    };
+    rc("rate_constant", x.rate_constant);
+    rc("k0", x.k0);
+    o << ",\"troe\":[" << num(x.troe.A) << "," << num(x.troe.T3) << "," << num(x.troe.T1) << "," << num(x.troe.T2) << "]";
+    o << ",\"efficiencies\":[";
+    for (size_t k = 0; k < x.efficiencies.size(); k++) o << (k ? "," : "") << num(x.efficiencies[k]);
+    o << "]}";
+  }
+  o << "]}";
+  return o.str();
+}
+
+cb_program* make_program(std::shared_ptr<const Mechanism> m, ast::Function f, ProgramKind kind) {
+  auto* p = new cb_program;
+  p->mechanism = std::move(m);
+  p->function = std::move(f);
+  p->linear = ast::linearize(p->function);
+  p->kind = kind;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cb_last_error(void) { return last_error.c_str(); }
+const char* cb_version(void) { return "combustion_amd 0.1 (gfx950)"; }
+
+static std::shared_ptr<const Mechanism> parse_mechanism(const std::string& text) {
+  try {
+    return std::make_shared<const Mechanism>(parse_yaml(text));
+  } catch (const yaml::Error&) {
+    throw;
+  } catch (const std::exception& e) {
+    throw yaml::Error(e.what());
+  }
+}
+
+int cb_mechanism_parse_yaml(const char* text, cb_mechanism** out) {
+  return guard([&] {
+    require(text && out, "null argument");
+    *out = new cb_mechanism{parse_mechanism(text)};
+  });
+}
+
+int cb_mechanism_load_yaml(const char* path, cb_mechanism** out) {
+  return guard([&] {
+    require(path && out, "null argument");
+    std::ifstream f(path);
+    if (!f) throw ArgumentError(std::string("cannot open ") + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    *out = new cb_mechanism{parse_mechanism(ss.str())};
+  });
+}
+
+void cb_mechanism_free(cb_mechanism* m) { delete m; }
+
+int cb_mechanism_sizes(const cb_mechanism* m, int32_t* species, int32_t* active, int32_t* reactions) {
+  return guard([&] {
+    require(m, "null mechanism");
+    if (species) *species = (int32_t)m->m->species.len();
+    if (active) *active = (int32_t)m->m->active;
+    if (reactions) *reactions = (int32_t)m->m->reactions.size();
+  });
+}
+
+int cb_mechanism_json(const cb_mechanism* m, char* buffer, size_t capacity, size_t* needed) {
+  int status = guard([&] { require(m, "null mechanism"); });
+  if (status) return status;
+  return write_text(mechanism_json(*m->m), buffer, capacity, needed);
+}
+
+int cb_transport_fits(const cb_mechanism* m, int32_t D, double T0, double* conductivity, double* viscosity, double* diffusivity) {
+  return guard([&] {
+    require(m && conductivity && viscosity && diffusivity, "null argument");
+    Polynomials p = fit_polynomials(m->m->species, T0, D);
+    std::copy(p.conductivityIVT.begin(), p.conductivityIVT.end(), conductivity);
+    std::copy(p.VviscosityIVVT.begin(), p.VviscosityIVVT.end(), viscosity);
+    std::copy(p.diffusivityITVT.begin(), p.diffusivityITVT.end(), diffusivity);
+  });
+}
+
+int cb_transport_species(const cb_mechanism* m, int32_t what, int32_t a, int32_t b, double T, double* out) {
+  return guard([&] {
+    require(m && out, "null argument");
+    const Species& s = m->m->species;
+    require(a >= 0 && (size_t)a < s.len() && b >= 0 && (size_t)b < s.len(), "species index out of range");
+    switch (what) {
+      case 0: *out = transport::viscosity(s, a, T); break;
+      case 1: *out = transport::conductivity(s, a, T); break;
+      case 2: *out = transport::diffusivity(s, a, b, T); break;
+      case 3: *out = transport::omega22(s, a, b, T); break;
+      case 4: *out = transport::omega11(s, a, b, T); break;
+      default: throw ArgumentError("unknown property");
+    }
+  });
+}
+
+int cb_transport_scales(const cb_mechanism* m, double T0, double* viscosity, double* conductivity) {
+  return guard([&] {
+    require(m && viscosity && conductivity, "null argument");
+    TransportScales s = transport_scales(*m->m, T0);
+    *viscosity = s.viscosity;
+    *conductivity = s.conductivity;
+  });
+}
+
+int cb_build_rates(const cb_mechanism* m, cb_program** out) {
+  return guard([&] {
+    require(m && out, "null argument");
+    *out = make_program(m->m, rates(*m->m), ProgramKind::Other);
+  });
+}
+
+int cb_build_reactions_rates(const cb_mechanism* m, cb_program** out) {
+  return guard([&] {
+    require(m && out, "null argument");
+    *out = make_program(m->m, reactions_rates(*m->m), ProgramKind::Other);
+  });
+}
+
+int cb_build_transport(const cb_mechanism* m, int32_t D, double T0, double viscosity, double conductivity, cb_program** out) {
+  return guard([&] {
+    require(m && out, "null argument");
+    Polynomials p = fit_polynomials(m->m->species, T0, D);
+    *out = make_program(m->m, properties(m->m->species, p, T0, viscosity, conductivity), ProgramKind::Other);
+  });
+}
+
+int cb_build_chemistry(const cb_mechanism* m, int32_t flags, int32_t D, double T0, double viscosity, double conductivity, cb_program** out) {
+  return guard([&] {
+    require(m && out, "null argument");
+    require(flags > 0 && flags < 8, "flags must combine CB_RATES, CB_TRANSPORT, CB_RHS");
+    ChemistryDescription d;
+    d.options.rates = flags & CB_RATES;
+    d.options.transport = flags & CB_TRANSPORT;
+    d.options.rhs = flags & CB_RHS;
+    d.scales = transport_scales(*m->m, T0);
+    if (viscosity > 0) d.scales.viscosity = viscosity;
+    if (conductivity > 0) d.scales.conductivity = conductivity;
+    if (d.options.transport) d.polynomials = std::make_shared<Polynomials>(fit_polynomials(m->m->species, T0, D));
+    ast::Function f = chemistry(*m->m, d.polynomials.get(), d.scales, d.options);
+    cb_program* p = make_program(m->m, std::move(f), ProgramKind::Chemistry);
+    p->chemistry = d;
+    *out = p;
+  });
+}
+
+void cb_program_free(cb_program* p) { delete p; }
+
+int cb_program_sizes(const cb_program* p, int32_t* inputs, int32_t* outputs, int64_t* statements, int64_t* instructions, int32_t* registers) {
+  return guard([&] {
+    require(p, "null program");
+    if (inputs) *inputs = p->function.n_input;
+    if (outputs) *outputs = (int32_t)p->function.output.size();
+    if (statements) *statements = (int64_t)p->function.statements.size();
+    if (instructions) *instructions = (int64_t)p->linear.code.size();
+    if (registers) *registers = p->linear.n_registers;
+  });
+}
+
+int cb_program_op_counts(const cb_program* p, int64_t counts[8]) {
+  return guard([&] {
+    require(p && counts, "null argument");
+    ast::OpCounts c = ast::count(p->linear);
+    int64_t v[8] = {c.alu, c.div, c.sqrt, c.exp, c.ln, c.neg, c.select, c.constants};
+    std::copy(v, v + 8, counts);
+  });
+}
+
+int cb_program_linear(const cb_program* p, uint8_t* op, int32_t* dst, int32_t* a, int32_t* b, int32_t* c, double* k, int32_t* outputs) {
+  return guard([&] {
+    require(p && op && dst && a && b && c && k && outputs, "null argument");
+    for (size_t i = 0; i < p->linear.code.size(); i++) {
+      const ast::Instruction& x = p->linear.code[i];
+      op[i] = (uint8_t)x.op;
+      dst[i] = x.dst;
+      a[i] = x.a;
+      b[i] = x.b;
+      c[i] = x.c;
+      k[i] = x.k;
+    }
+    std::copy(p->linear.output.begin(), p->linear.output.end(), outputs);
+  });
+}
+
+int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed) {
+  std::string text;
+  int status = guard([&] {
+    require(p, "null program");
+    ProgramView v{p->mechanism, &p->function, &p->linear, p->kind, &p->chemistry};
+    text = form_source(v, form, block_size);
+  });
+  if (status) return status;
+  return write_text(text, buffer, capacity, needed);
+}
+
+int cb_hip_compile(const cb_program* p, int32_t form, uint32_t block_size) {
+  return guard([&] {
+    require(p, "null program");
+    ProgramView v{p->mechanism, &p->function, &p->linear, p->kind, &p->chemistry};
+    compile_form(v, form, block_size);
+  });
+}
+
+int cb_hip_assemble(const cb_program* p, int32_t form, uint32_t block_size, cb_module** out) {
+  return guard([&] {
+    require(p && out, "null argument");
+    ProgramView v{p->mechanism, &p->function, &p->linear, p->kind, &p->chemistry};
+    auto* m = new cb_module;
+    try {
+      m->form = assemble_form(v, form, block_size);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs, int32_t n_outputs,
+                int32_t pointers_are_device, void* stream, float* kernel_ms) {
+  return guard([&] {
+    require(mod && mod->form, "null module");
+    require(n_inputs == mod->form->n_inputs() && n_outputs == mod->form->n_outputs(), "input/output count does not match the program");
+    require(n_states == 0 || (inputs && outputs), "null array table");
+    for (int32_t i = 0; i < n_inputs && n_states; i++) require(inputs[i], "null input array");
+    for (int32_t i = 0; i < n_outputs && n_states; i++) require(outputs[i], "null output array");
+    require(n_states < (size_t(1) << 31), "n_states must be below 2^31");
+    mod->form->call(inputs, n_states, outputs, pointers_are_device != 0, (hipStream_t)stream, kernel_ms);
+  });
+}
+
+void cb_module_free(cb_module* mod) { delete mod; }
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Kernel forms behind cb_hip_assemble: each turns a program (and, for the tiled forms, the mechanism it was
+// built from) into gfx950 code and launches it over SoA state arrays.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "ast.hpp"
+#include "builders.hpp"
+#include "mechanism.hpp"
+
+namespace combustion {
+
+struct ArgumentError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct CompileError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+enum class ProgramKind { Other, Chemistry };
+
+struct ChemistryDescription {
+  ChemistryOptions options;
+  TransportScales scales{};
+  std::shared_ptr<Polynomials> polynomials;
+};
+
+struct ProgramView {
+  std::shared_ptr<const Mechanism> mechanism;
+  const ast::Function* function;
+  const ast::Linear* linear;
+  ProgramKind kind;
+  const ChemistryDescription* chemistry;
+};
+
+class Form {
+ public:
+  virtual ~Form() = default;
+  virtual int n_inputs() const = 0;
+  virtual int n_outputs() const = 0;
+  // inputs/outputs: tables of n_inputs()/n_outputs() arrays of n doubles, host or device memory.
+  virtual void call(const double* const* inputs, size_t n, double* const* outputs, bool device, hipStream_t stream, float* kernel_ms) = 0;
+};
+
+// Common hiprtc options for gfx950 (packed FP32 instructions disabled).
+std::vector<std::string> hip_options(bool contract);
+
+std::string form_source(const ProgramView& p, int form, unsigned block_size);
+void compile_form(const ProgramView& p, int form, unsigned block_size);
+std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size);
+
+// Tiled forms (tiled.cpp).
+std::string tiled_source(const ProgramView& p, bool mixed, unsigned block_size);
+void compile_tiled(const ProgramView& p, bool mixed, unsigned block_size);
+std::unique_ptr<Form> assemble_tiled(const ProgramView& p, bool mixed, unsigned block_size);
+
+}  // namespace combustion

@@ -1,0 +1,282 @@
+"""ctypes binding of include/combustion_amd.h (libcombustion_amd.so, built in place by combustion_amd.build).
+
+Mirrors the reference's host API on top of the C ABI:
+  parse/new            yaml.rs:54 + lib.rs:127        -> Mechanism(path or text)
+  reaction::rates      reaction.rs:356                -> Mechanism.rates()
+  transport::properties transport.rs:212              -> Mechanism.transport(D, T0, viscosity, conductivity)
+  device::assemble     device.rs:25,75                -> Program.assemble(block_size, form) -> Module
+  (the returned Fn)    device.rs:59,98-100            -> Module(inputs) -> outputs
+The library is loaded once; a missing library raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcombustion_amd.so")
+
+RATES, TRANSPORT, RHS = 1, 2, 4
+FORM_STRAIGHT, FORM_FP64, FORM_MIXED = 0, 1, 2
+FORMS = {"straight": FORM_STRAIGHT, "fp64": FORM_FP64, "mixed": FORM_MIXED}
+
+STATUS = {0: "ok", -1: "argument", -2: "parse", -3: "unsupported", -4: "build", -5: "compile", -6: "device", -7: "size"}
+
+
+class Error(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+        self.kind = STATUS.get(status, str(status))
+
+
+_lib = None
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+_DPP = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cb_last_error": (C.c_char_p, []),
+    "cb_version": (C.c_char_p, []),
+    "cb_mechanism_load_yaml": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "cb_mechanism_parse_yaml": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "cb_mechanism_free": (None, [_P]),
+    "cb_mechanism_sizes": (C.c_int, [_P, _I32P, _I32P, _I32P]),
+    "cb_mechanism_json": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cb_transport_fits": (C.c_int, [_P, C.c_int32, C.c_double, _DP, _DP, _DP]),
+    "cb_transport_species": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_double, _DP]),
+    "cb_transport_scales": (C.c_int, [_P, C.c_double, _DP, _DP]),
+    "cb_build_rates": (C.c_int, [_P, C.POINTER(_P)]),
+    "cb_build_reactions_rates": (C.c_int, [_P, C.POINTER(_P)]),
+    "cb_build_transport": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
+    "cb_build_chemistry": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
+    "cb_program_free": (None, [_P]),
+    "cb_program_sizes": (C.c_int, [_P, _I32P, _I32P, _I64P, _I64P, _I32P]),
+    "cb_program_op_counts": (C.c_int, [_P, _I64P]),
+    "cb_program_linear": (C.c_int, [_P, C.POINTER(C.c_uint8), _I32P, _I32P, _I32P, _I32P, _DP, _I32P]),
+    "cb_program_hip_source": (C.c_int, [_P, C.c_int32, C.c_uint32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cb_hip_assemble": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
+    "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
+    "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
+    "cb_module_free": (None, [_P]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m combustion_amd.build`")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise Error(status, lib().cb_last_error().decode())
+
+
+def _text(fn, *args):
+    needed = C.c_size_t(0)
+    status = fn(*args, None, 0, C.byref(needed))
+    if status not in (0, -7):
+        _check(status)
+    buf = C.create_string_buffer(needed.value)
+    _check(fn(*args, buf, needed.value, C.byref(needed)))
+    return buf.value.decode()
+
+
+class Mechanism:
+    """yaml::parse + combustion::new (yaml.rs:54, lib.rs:127)."""
+
+    def __init__(self, path=None, text=None):
+        h = _P()
+        if text is not None:
+            _check(lib().cb_mechanism_parse_yaml(text.encode(), C.byref(h)))
+        else:
+            _check(lib().cb_mechanism_load_yaml(os.fsencode(path), C.byref(h)))
+        self._h = h
+        K, A, R = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().cb_mechanism_sizes(h, C.byref(K), C.byref(A), C.byref(R)))
+        self.K, self.active, self.R = K.value, A.value, R.value
+        self._json = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cb_mechanism_free(self._h)
+            self._h = None
+
+    @property
+    def tables(self):
+        """Parsed species and reaction tables in SI units (dict from the library's JSON dump)."""
+        if self._json is None:
+            self._json = json.loads(_text(lib().cb_mechanism_json, self._h))
+        return self._json
+
+    @property
+    def species_names(self):
+        return [s["name"] for s in self.tables["species"]]
+
+    @property
+    def molar_mass(self):
+        return np.array([s["molar_mass"] for s in self.tables["species"]])
+
+    def transport_fits(self, D=4, T0=1000.0):
+        K = self.K
+        c, v, d = np.zeros((K, D)), np.zeros((K, D)), np.zeros((K, K, D))
+        _check(lib().cb_transport_fits(self._h, D, T0, c.ctypes.data_as(_DP), v.ctypes.data_as(_DP), d.ctypes.data_as(_DP)))
+        return c, v, d
+
+    def transport_species(self, what, a, b, T):
+        out = C.c_double()
+        code = {"viscosity": 0, "conductivity": 1, "diffusivity": 2, "omega22": 3, "omega11": 4}[what]
+        _check(lib().cb_transport_species(self._h, code, a, b, T, C.byref(out)))
+        return out.value
+
+    def transport_scales(self, T0=1000.0):
+        v, c = C.c_double(), C.c_double()
+        _check(lib().cb_transport_scales(self._h, T0, C.byref(v), C.byref(c)))
+        return T0, v.value, c.value
+
+    def rates(self):
+        """reaction::rates (reaction.rs:356)."""
+        h = _P()
+        _check(lib().cb_build_rates(self._h, C.byref(h)))
+        return Program(h, self)
+
+    def reactions_rates(self):
+        h = _P()
+        _check(lib().cb_build_reactions_rates(self._h, C.byref(h)))
+        return Program(h, self)
+
+    def transport(self, D=5, T0=1000.0, viscosity=1.0, conductivity=1.0):
+        """transport::properties::<D> (transport.rs:212)."""
+        h = _P()
+        _check(lib().cb_build_transport(self._h, D, T0, viscosity, conductivity, C.byref(h)))
+        return Program(h, self)
+
+    def chemistry(self, flags=RATES | TRANSPORT, D=4, T0=1000.0, viscosity=0.0, conductivity=0.0):
+        """Per-state SoA program: inputs [T, P, Y_0..Y_{K-1}] -> [ω_0..ω_{Ka-1}, λ, μ, D_0..D_{K-1}, (dT/dt)]."""
+        h = _P()
+        _check(lib().cb_build_chemistry(self._h, flags, D, T0, viscosity, conductivity, C.byref(h)))
+        return Program(h, self)
+
+
+class Program:
+    """An ast::Function (ast/lib.rs:328) built by the library."""
+
+    def __init__(self, handle, mechanism):
+        self._h = handle
+        self.mechanism = mechanism
+        n_in, n_out, n_reg = C.c_int32(), C.c_int32(), C.c_int32()
+        n_st, n_ins = C.c_int64(), C.c_int64()
+        _check(lib().cb_program_sizes(handle, C.byref(n_in), C.byref(n_out), C.byref(n_st), C.byref(n_ins), C.byref(n_reg)))
+        self.n_inputs, self.n_outputs = n_in.value, n_out.value
+        self.n_statements, self.n_instructions, self.n_registers = n_st.value, n_ins.value, n_reg.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cb_program_free(self._h)
+            self._h = None
+
+    def op_counts(self):
+        a = (C.c_int64 * 8)()
+        _check(lib().cb_program_op_counts(self._h, a))
+        keys = ["alu", "div", "sqrt", "exp", "ln", "neg", "select", "constants"]
+        d = dict(zip(keys, list(a)))
+        d["weighted"] = d["alu"] + 20 * (d["exp"] + d["ln"]) + 8 * (d["div"] + d["sqrt"])
+        return d
+
+    def linear(self):
+        """Linear SSA form (op, dst, a, b, c, k, outputs) as numpy arrays."""
+        n = self.n_instructions
+        op = np.zeros(n, np.uint8)
+        dst, a, b, c = (np.zeros(n, np.int32) for _ in range(4))
+        k = np.zeros(n)
+        out = np.zeros(self.n_outputs, np.int32)
+        _check(lib().cb_program_linear(self._h, op.ctypes.data_as(C.POINTER(C.c_uint8)), dst.ctypes.data_as(_I32P),
+                                       a.ctypes.data_as(_I32P), b.ctypes.data_as(_I32P), c.ctypes.data_as(_I32P),
+                                       k.ctypes.data_as(_DP), out.ctypes.data_as(_I32P)))
+        return {"op": op, "dst": dst, "a": a, "b": b, "c": c, "k": k, "outputs": out, "n_inputs": self.n_inputs,
+                "n_registers": self.n_registers}
+
+    def hip_source(self, form="straight", block_size=0):
+        return _text(lib().cb_program_hip_source, self._h, FORMS[form], block_size)
+
+    def compile(self, form="straight", block_size=0):
+        """Fills the code-object cache for a form (hiprtc for gfx950, no GPU needed)."""
+        _check(lib().cb_hip_compile(self._h, FORMS[form], block_size))
+
+    def assemble(self, block_size=0, form="straight"):
+        """device::assemble (device.rs:25): emit, compile with hiprtc, load the module."""
+        h = _P()
+        _check(lib().cb_hip_assemble(self._h, FORMS[form], block_size, C.byref(h)))
+        return Module(h, self)
+
+
+def _pointer(x):
+    if isinstance(x, np.ndarray):
+        if x.dtype != np.float64 or not x.flags.c_contiguous:
+            raise ValueError("arrays must be contiguous float64")
+        return x.ctypes.data, False
+    # torch tensor (device memory)
+    if getattr(x, "is_cuda", False):
+        if str(x.dtype) != "torch.float64" or not x.is_contiguous():
+            raise ValueError("tensors must be contiguous float64")
+        return x.data_ptr(), True
+    raise TypeError("inputs must be numpy arrays or torch device tensors")
+
+
+class Module:
+    """The callable device::assemble returns (device.rs:59): module(inputs) -> outputs (SoA)."""
+
+    def __init__(self, handle, program):
+        self._h = handle
+        self.program = program
+        self.last_kernel_ms = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cb_module_free(self._h)
+            self._h = None
+
+    def __call__(self, inputs, outputs=None, stream=None, time_kernel=False):
+        """inputs: sequence of n_inputs arrays of N states (numpy = host, torch cuda = device).
+        Returns outputs (allocated like the inputs when not given)."""
+        if len(inputs) != self.program.n_inputs:
+            raise ValueError(f"expected {self.program.n_inputs} inputs, got {len(inputs)}")
+        n = len(inputs[0])
+        ptrs = [_pointer(x) for x in inputs]
+        on_device = ptrs[0][1]
+        if any(d != on_device for _, d in ptrs):
+            raise ValueError("inputs must all be on the host or all on the device")
+        if any(len(x) != n for x in inputs):
+            raise ValueError("all inputs must have the same length")
+        if outputs is None:
+            if on_device:
+                import torch
+                outputs = [torch.empty(n, dtype=torch.float64, device=inputs[0].device) for _ in range(self.program.n_outputs)]
+            else:
+                outputs = [np.empty(n) for _ in range(self.program.n_outputs)]
+        optrs = [_pointer(x) for x in outputs]
+        if any(d != on_device for _, d in optrs) or len(outputs) != self.program.n_outputs:
+            raise ValueError("outputs must match the inputs' placement and the program's output count")
+        in_arr = (C.c_void_p * len(ptrs))(*[p for p, _ in ptrs])
+        out_arr = (C.c_void_p * len(optrs))(*[p for p, _ in optrs])
+        ms = C.c_float(0)
+        _check(lib().cb_hip_call(self._h, in_arr, len(ptrs), n, out_arr, len(optrs), int(on_device), stream,
+                                 C.byref(ms) if time_kernel else None))
+        if time_kernel:
+            self.last_kernel_ms = ms.value
+        return outputs

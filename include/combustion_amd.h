@@ -1,0 +1,107 @@
+/* C ABI of the MI355X chemistry backend.
+ *
+ * The reference builds one straight-line program per mechanism (ast::Function) and hands it to a launcher:
+ *   device::assemble(function, block_size) -> Fn(constants, SoA inputs) -> SoA outputs   (device.rs:25,75,98-100)
+ * This header is that boundary for HIP on gfx950: the builders (yaml::parse, combustion::new, reaction::rates,
+ * transport::properties) produce a cb_program, cb_hip_assemble compiles it with hiprtc into a cb_module, and
+ * cb_hip_call runs it over N states in SoA layout. Plain pointers and sizes only.
+ *
+ * Conventions: every function returns CB_OK (0) or a negative status and never aborts; cb_last_error() then
+ * holds a message for the calling thread. The caller owns every buffer (the reference returns freshly
+ * allocated outputs, device.rs:34,72; here outputs are caller-allocated arrays of n_states doubles).
+ * A cb_module is used by one host thread at a time; modules are independent.
+ */
+#ifndef COMBUSTION_AMD_H
+#define COMBUSTION_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cb_mechanism cb_mechanism;
+typedef struct cb_program cb_program;
+typedef struct cb_module cb_module;
+
+enum cb_status {
+  CB_OK = 0,
+  CB_ERR_ARGUMENT = -1,    /* null pointer, size mismatch (device.rs:31-32 asserts) */
+  CB_ERR_PARSE = -2,       /* malformed mechanism file (yaml.rs panics) */
+  CB_ERR_UNSUPPORTED = -3, /* e.g. SRI falloff (reaction.rs:194-203 unimplemented!) */
+  CB_ERR_BUILD = -4,       /* builder contract violation (ast/lib.rs asserts, f32 overflow of a constant) */
+  CB_ERR_COMPILE = -5,     /* hiprtc failure */
+  CB_ERR_DEVICE = -6,      /* HIP runtime failure (device.rs anyhow::Result errors) */
+  CB_ERR_SIZE = -7         /* caller buffer too small; *needed says how much */
+};
+
+/* Program forms for cb_hip_assemble. */
+enum cb_form {
+  CB_FORM_STRAIGHT = 0, /* the program emitted as straight-line HIP, one lane per state (cuda.rs:101-134) */
+  CB_FORM_FP64 = 1,     /* mechanism-tiled FP64 kernels, constants in constant memory (chemistry programs) */
+  CB_FORM_MIXED = 2     /* tiled mixed precision: BF16-split MFMA contractions, FP32 exp/log, FP64 ΔG/RT */
+};
+
+/* Sections of the per-state program built by cb_build_chemistry. */
+enum cb_chemistry_flags { CB_RATES = 1, CB_TRANSPORT = 2, CB_RHS = 4 };
+
+const char* cb_last_error(void);
+const char* cb_version(void);
+
+/* yaml.rs:54 parse + lib.rs:127 new: Cantera-YAML (cm/s/mol units) to species and reaction tables. */
+int cb_mechanism_load_yaml(const char* path, cb_mechanism** out);
+int cb_mechanism_parse_yaml(const char* text, cb_mechanism** out);
+void cb_mechanism_free(cb_mechanism* m);
+/* K species, the first `active` of which react (lib.rs:130-136), and R reactions. */
+int cb_mechanism_sizes(const cb_mechanism* m, int32_t* species, int32_t* active, int32_t* reactions);
+/* JSON dump of the parsed tables in SI units (lib.rs:26-125 fields), for inspection and tests. */
+int cb_mechanism_json(const cb_mechanism* m, char* buffer, size_t capacity, size_t* needed);
+
+/* transport.rs:125-136 Polynomials::<D>::new: fits in ln(T/T0), conductivity[K][D], viscosity[K][D],
+ * diffusivity[K][K][D] (entries j<k fitted, j>k mirrored). */
+int cb_transport_fits(const cb_mechanism* m, int32_t D, double T0, double* conductivity, double* viscosity, double* diffusivity);
+/* transport.rs:97-116 single-species physics at T: what = 0 viscosity(a), 1 conductivity(a),
+ * 2 diffusivity(a,b), 3 Ω*22(a,b), 4 Ω*11(a,b). */
+int cb_transport_species(const cb_mechanism* m, int32_t what, int32_t a, int32_t b, double T, double* out);
+/* Reference scales used by cb_build_chemistry (the role of cantera.rs:60-76). */
+int cb_transport_scales(const cb_mechanism* m, double T0, double* viscosity, double* conductivity);
+
+/* reaction.rs:356 rates: inputs [P/R, R/P, T, V, n_0..n_{K-2}], outputs [T dT/dt, dV/dt, ω_0..ω_{Ka-1}]. */
+int cb_build_rates(const cb_mechanism* m, cb_program** out);
+/* reaction.rs:329 reactions_rates: inputs [P/R, T, V, n_0..n_{K-2}], outputs one rate of progress per reaction. */
+int cb_build_reactions_rates(const cb_mechanism* m, cb_program** out);
+/* transport.rs:212 properties::<D>: inputs [Σx, T/T0, x_0..x_{K-2}], outputs [λ/λ0, μ/η0, ρD_0/η0..]. */
+int cb_build_transport(const cb_mechanism* m, int32_t D, double T0, double viscosity, double conductivity, cb_program** out);
+/* Per-state SoA program (the GPU boundary's shape, cuda.rs:210-226 family): inputs [T, P, Y_0..Y_{K-1}]
+ * (K, Pa, mass fractions); outputs [ω_0..ω_{Ka-1}] if CB_RATES, [λ, μ, D_0..D_{K-1}] if CB_TRANSPORT,
+ * [dT/dt] if CB_RHS, in that order. viscosity/conductivity <= 0 select cb_transport_scales. */
+int cb_build_chemistry(const cb_mechanism* m, int32_t flags, int32_t D, double T0, double viscosity, double conductivity, cb_program** out);
+void cb_program_free(cb_program* p);
+int cb_program_sizes(const cb_program* p, int32_t* inputs, int32_t* outputs, int64_t* statements, int64_t* instructions, int32_t* registers);
+/* cuda.rs:229-254 categories over the linear program: alu, div, sqrt, exp, ln, neg, select, constants. */
+int cb_program_op_counts(const cb_program* p, int64_t counts[8]);
+/* Linear SSA form of the program (ast/interpret.rs semantics, one instruction per node): op codes
+ * 0 const,1 neg,2 min,3 max,4 add,5 sub,6 le,7 mul,8 div,9 sqrt,10 exp,11 ln,12 sq,13 select(a ? b : c).
+ * Arrays have `instructions` entries; outputs has `outputs` entries (registers). */
+int cb_program_linear(const cb_program* p, uint8_t* op, int32_t* dst, int32_t* a, int32_t* b, int32_t* c, double* k, int32_t* outputs);
+/* HIP text of a form (cuda.rs:101 compile counterpart). */
+int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed);
+
+/* device.rs:75 assemble: emit + hiprtc(gfx950) + module load. block_size 0 picks the form's default. */
+int cb_hip_assemble(const cb_program* p, int32_t form, uint32_t block_size, cb_module** out);
+/* Compiles a form into the code-object cache without touching a GPU. */
+int cb_hip_compile(const cb_program* p, int32_t form, uint32_t block_size);
+/* device.rs:59 call: inputs[n_inputs] and outputs[n_outputs] are SoA arrays of n_states doubles, host or
+ * device memory (pointers_are_device). stream is a hipStream_t or NULL. With host pointers the call blocks;
+ * with device pointers it is asynchronous unless kernel_ms is requested (then it waits and reports the
+ * kernel time measured with HIP events on that stream). */
+int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs, int32_t n_outputs,
+                int32_t pointers_are_device, void* stream, float* kernel_ms);
+void cb_module_free(cb_module* mod);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
