@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--form", default="auto")
     ap.add_argument("--states", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block-size", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every state is independent, so N GPUs would each run "
@@ -82,13 +83,13 @@ def main():
     if form == "auto":
         form = "fp64"
         try:
-            mod = prog.assemble(form=form)
-        except Exception as e:  # the tiled form is required for the headline number; report what ran
+            mod = prog.assemble(args.block_size, form=form)
+        except Exception as e:
             log("fp64 form unavailable:", e)
             form = "straight"
-            mod = prog.assemble(form=form)
+            mod = prog.assemble(args.block_size, form=form)
     else:
-        mod = prog.assemble(form=form)
+        mod = prog.assemble(args.block_size, form=form)
     N = args.states
     T, P, Y = synth.config_states(MECHANISM, N, 2024)
     inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]

@@ -2,6 +2,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <functional>
 
 namespace combustion::ast {
@@ -182,7 +183,33 @@ Linear linearize(const Function& f) {
     node_reg[e] = r;
     return r;
   };
-  for (const Statement& s : f.statements) {
+  // Each output is evaluated right after the last statement it depends on, so that the values only it
+  // needs (e.g. the viscosity sums) do not stay live through later statements.
+  std::vector<int32_t> defined_at(f.values.size(), -1);
+  for (size_t i = 0; i < f.statements.size(); i++) {
+    const Statement& s = f.statements[i];
+    if (!s.select) defined_at[s.id] = (int32_t)i;
+    for (int32_t r : s.results) defined_at[r] = (int32_t)i;
+  }
+  std::vector<int32_t> ready(f.nodes.size(), -2);  // latest statement index a node depends on
+  std::function<int32_t(E)> depends = [&](E e) -> int32_t {
+    if (ready[e] != -2) return ready[e];
+    const Node& n = f.nodes[e];
+    int32_t d = -1;
+    if (n.op == Op::Value)
+      d = defined_at[n.a];
+    else if (n.op != Op::Const) {
+      d = depends(n.a);
+      if (n.b >= 0) d = std::max(d, depends(n.b));
+    }
+    return ready[e] = d;
+  };
+  std::vector<std::vector<size_t>> outputs_after(f.statements.size() + 1);
+  for (size_t o = 0; o < f.output.size(); o++) outputs_after[depends(f.output[o]) + 1].push_back(o);
+  l.output.assign(f.output.size(), -1);
+  for (size_t o : outputs_after[0]) l.output[o] = eval(f.output[o]);
+  for (size_t i = 0; i < f.statements.size(); i++) {
+    const Statement& s = f.statements[i];
     if (!s.select) {
       value_reg[s.id] = eval(s.value);
     } else {
@@ -193,8 +220,8 @@ Linear linearize(const Function& f) {
         value_reg[s.results[i]] = emit(LOp::Select, cond, t, fr, 0);
       }
     }
+    for (size_t o : outputs_after[i + 1]) l.output[o] = eval(f.output[o]);
   }
-  for (E e : f.output) l.output.push_back(eval(e));
   l.n_registers = next;
   return l;
 }

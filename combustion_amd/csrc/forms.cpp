@@ -1,5 +1,6 @@
 #include "forms.hpp"
 
+#include <cstdlib>
 #include <vector>
 
 #include "device.hpp"
@@ -19,9 +20,9 @@ namespace {
 // every operation rounds as in the FP64 interpreter (ast/interpret.rs).
 class Straight : public SoAForm {
  public:
-  Straight(const ProgramView& p, unsigned block_size)
-      : SoAForm(p.function->n_input, (int)p.function->output.size()), block_size_(block_size) {
-    kernel_.load(device::compile(emit_straight(*p.linear, "program", block_size), "program", hip_options(false)), "program");
+  Straight(const ProgramView& p, const StraightOptions& o, bool contract)
+      : SoAForm(p.function->n_input, (int)p.function->output.size()), block_size_(o.block_size) {
+    kernel_.load(device::compile(emit_straight(*p.linear, "program", o), "program", hip_options(contract)), "program");
   }
 
  protected:
@@ -39,34 +40,41 @@ class Straight : public SoAForm {
 
 }  // namespace
 
-std::string form_source(const ProgramView& p, int form, unsigned block_size) {
-  switch (form) {
-    case 0: return emit_straight(*p.linear, "program", block_size ? block_size : 256);
-    case 1: return tiled_source(p, false, block_size);
-    case 2: return tiled_source(p, true, block_size);
+// Form 0 keeps every operation as the FP64 interpreter rounds it; form 1 on a straight-line program allows FMA
+// contraction and reciprocal-multiply division (both within an ulp or two per operation).
+StraightOptions straight_options(int form, unsigned block_size) {
+  StraightOptions o;
+  o.block_size = block_size ? block_size : 256;
+  if (form == 1) {
+    o.reciprocal = true;
+    if (const char* w = std::getenv("COMBUSTION_AMD_WAVES_PER_EU")) o.waves_per_eu = (unsigned)std::atoi(w);
   }
-  throw ArgumentError("unknown form");
+  return o;
+}
+
+bool use_tiled(const ProgramView& p, int form) {
+  if (form == 2) return true;
+  if (form != 1) return false;
+  if (const char* t = std::getenv("COMBUSTION_AMD_FP64_TILED")) return std::atoi(t) != 0;
+  return false;
+}
+
+std::string form_source(const ProgramView& p, int form, unsigned block_size) {
+  if (form < 0 || form > 2) throw ArgumentError("unknown form");
+  if (use_tiled(p, form)) return tiled_source(p, form == 2, block_size);
+  return emit_straight(*p.linear, "program", straight_options(form, block_size));
 }
 
 void compile_form(const ProgramView& p, int form, unsigned block_size) {
-  if (form == 0) {
-    device::compile(emit_straight(*p.linear, "program", block_size ? block_size : 256), "program", hip_options(false));
-    return;
-  }
-  if (form == 1 || form == 2) {
-    compile_tiled(p, form == 2, block_size);
-    return;
-  }
-  throw ArgumentError("unknown form");
+  if (form < 0 || form > 2) throw ArgumentError("unknown form");
+  if (use_tiled(p, form)) return compile_tiled(p, form == 2, block_size);
+  device::compile(emit_straight(*p.linear, "program", straight_options(form, block_size)), "program", hip_options(form == 1));
 }
 
 std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size) {
-  switch (form) {
-    case 0: return std::make_unique<Straight>(p, block_size ? block_size : 256);
-    case 1: return assemble_tiled(p, false, block_size);
-    case 2: return assemble_tiled(p, true, block_size);
-  }
-  throw ArgumentError("unknown form");
+  if (form < 0 || form > 2) throw ArgumentError("unknown form");
+  if (use_tiled(p, form)) return assemble_tiled(p, form == 2, block_size);
+  return std::make_unique<Straight>(p, straight_options(form, block_size), form == 1);
 }
 
 }  // namespace combustion

@@ -18,15 +18,34 @@ std::string literal(double x) {
 }
 }  // namespace
 
-std::string emit_straight(const ast::Linear& l, const std::string& name, unsigned block_size) {
+std::string emit_straight(const ast::Linear& l, const std::string& name, const StraightOptions& o) {
   using ast::LOp;
   std::ostringstream s;
-  s << "extern \"C\" __global__ void __launch_bounds__(" << block_size << ") " << name
+  if (o.reciprocal && !o.f32)
+    s << "__device__ __forceinline__ double rcp64(double b) {\n"
+         "  double r = __builtin_amdgcn_rcp(b);\n"
+         "  double e = __builtin_fma(-b, r, 1.0);\n"
+         "  r = __builtin_fma(r, e, r);\n"
+         "  e = __builtin_fma(-b, r, 1.0);\n"
+         "  return __builtin_fma(r, e, r);\n"
+         "}\n";
+  const char* F = o.f32 ? "float" : "double";
+  if (o.f32)
+    s << "__device__ __forceinline__ float rcp32(float b) { return __builtin_amdgcn_rcpf(b); }\n"
+         "__device__ __forceinline__ float exp32(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }\n"
+         "__device__ __forceinline__ float log32(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }\n";
+  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block_size << ") ";
+  if (o.waves_per_eu) s << "__attribute__((amdgpu_waves_per_eu(" << o.waves_per_eu << ", " << o.waves_per_eu << "))) ";
+  s << name
     << "(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n, unsigned offset) {\n"
     << "  const unsigned id = offset + blockIdx.x * blockDim.x + threadIdx.x;\n"
     << "  if (id >= n) return;\n";
-  for (int i = 0; i < l.n_input; i++) s << "  const double r" << i << " = in[" << i << "][id];\n";
+  for (int i = 0; i < l.n_input; i++) s << "  const " << F << " r" << i << " = (" << F << ")in[" << i << "][id];\n";
   std::vector<bool> is_bool(l.n_registers, false);
+  // Outputs are stored as soon as their register is computed, ending their live range early.
+  std::vector<std::vector<size_t>> stores(l.n_registers);
+  for (size_t o = 0; o < l.output.size(); o++)
+    if (l.output[o] >= l.n_input) stores[l.output[o]].push_back(o);
   for (const ast::Instruction& i : l.code) {
     const std::string d = "r" + std::to_string(i.dst), a = "r" + std::to_string(i.a), b = "r" + std::to_string(i.b),
                       c = "r" + std::to_string(i.c);
@@ -35,26 +54,35 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, unsigne
       s << "  const bool " << d << " = " << a << " <= " << b << ";\n";
       continue;
     }
-    s << "  const double " << d << " = ";
+    s << "  const " << F << " " << d << " = ";
     switch (i.op) {
-      case LOp::Const: s << literal(i.k); break;
+      case LOp::Const: s << literal(i.k) << (o.f32 ? "f" : ""); break;
       case LOp::Neg: s << "-" << a; break;
-      case LOp::Min: s << "fmin(" << a << ", " << b << ")"; break;
-      case LOp::Max: s << "fmax(" << a << ", " << b << ")"; break;
+      case LOp::Min: s << (o.f32 ? "fminf(" : "fmin(") << a << ", " << b << ")"; break;
+      case LOp::Max: s << (o.f32 ? "fmaxf(" : "fmax(") << a << ", " << b << ")"; break;
       case LOp::Add: s << a << " + " << b; break;
       case LOp::Sub: s << a << " - " << b; break;
       case LOp::Mul: s << a << " * " << b; break;
-      case LOp::Div: s << a << " / " << b; break;
-      case LOp::Sqrt: s << "sqrt(" << a << ")"; break;
-      case LOp::Exp: s << "exp(" << a << ")"; break;
-      case LOp::Ln: s << "log(" << a << ")"; break;
+      case LOp::Div:
+        if (o.f32)
+          s << a << " * rcp32(" << b << ")";
+        else if (o.reciprocal)
+          s << a << " * rcp64(" << b << ")";
+        else
+          s << a << " / " << b;
+        break;
+      case LOp::Sqrt: s << (o.f32 ? "__builtin_sqrtf(" : "sqrt(") << a << ")"; break;
+      case LOp::Exp: s << (o.f32 ? "exp32(" : "exp(") << a << ")"; break;
+      case LOp::Ln: s << (o.f32 ? "log32(" : "log(") << a << ")"; break;
       case LOp::Sq: s << a << " * " << a; break;
       case LOp::Select: s << a << " ? " << b << " : " << c; break;
       default: break;
     }
     s << ";\n";
+    for (size_t o : stores[i.dst]) s << "  out[" << o << "][id] = (double)" << d << ";\n";
   }
-  for (size_t o = 0; o < l.output.size(); o++) s << "  out[" << o << "][id] = r" << l.output[o] << ";\n";
+  for (size_t o = 0; o < l.output.size(); o++)
+    if (l.output[o] < l.n_input) s << "  out[" << o << "][id] = r" << l.output[o] << ";\n";
   s << "}\n";
   return s.str();
 }

@@ -93,7 +93,8 @@ def load(path=None, text=None):
     names = [s[0] for s in species]
     K = len(names)
     Ka = sum(reacts(n) for n in names)
-    W = np.array([sum(n * ATOMIC_WEIGHTS[e] for e, n in s["composition"].items()) / 1e3 for _, s, _, _ in species])
+    # lib.rs:23,42: each atomic weight in kg/mol, summed in composition order
+    W = np.array([sum(n * (ATOMIC_WEIGHTS[e] / 1e3) for e, n in s["composition"].items()) for _, s, _, _ in species])
     composition = np.array([[s["composition"].get(e, 0) for e in ELEMENTS] for _, s, _, _ in species])
     split = np.array([r[1] if len(r) == 3 else math.nan for _, _, r, _ in species])
     nasa = np.array([[p[0], p[1] if len(p) == 2 else p[0]] for _, _, _, p in species])

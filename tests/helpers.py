@@ -22,3 +22,13 @@ def setup(name, D=4):
 def relative_error(x, ref, floor):
     """|x - r| / max(|r|, floor): relative error with a per-output floor (the term scale for ω)."""
     return np.abs(x - ref) / np.maximum(np.abs(ref), floor)
+
+
+def state_floor(scale, Ka, ratio=1e-30):
+    """Mixed-form floor for ω: max(term scale s_k, ratio · max_j s_j) per state. The mixed form accumulates
+    ν·q in FP32 under one exponent per state, so rates more than ~1e38 below the state's largest term cannot
+    be represented; the stated floor puts them below 1e-30 of the state's largest term scale."""
+    import numpy as np
+    f = scale.copy()
+    f[:Ka] = np.maximum(scale[:Ka], ratio * scale[:Ka].max(axis=0, keepdims=True))
+    return f

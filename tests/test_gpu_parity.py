@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 from combustion_amd import synth
-from helpers import relative_error, setup
+from helpers import relative_error, setup, state_floor
 
 TOL = 1e-12
 
@@ -51,3 +51,37 @@ def test_host_and_device_pointers_agree(gpu):
     host = np.stack(prog.assemble(form="straight")([T, P, *np.ascontiguousarray(Y)]))
     dev = run(prog, "straight", T, P, Y, gpu)
     assert np.array_equal(host, dev)
+
+
+# Mixed form: split-BF16 MFMA contractions, FP32 exp/log, FP64 ΔG/RT and forward-minus-reverse subtraction.
+# Bars: ω_k within 1e-5 of the FP64 oracle relative to max(|ω_k|, s_k, 1e-30 max_j s_j) with the term scale
+# s_k = Σ_r |ν_kr| k_f (R_f + R_r) (see helpers.state_floor); λ, μ, D_k within 1e-3.
+MIXED_RATES_TOL = 1e-5
+MIXED_TRANSPORT_TOL = 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,n", [("h2o2-10", 4096), ("gri-53", 16384)])
+def test_mixed_matches_oracle_within_stated_error(name, n, gpu):
+    M, mech, o = setup(name)
+    T, P, Y = synth.config_states(name, n, 8)
+    ref, floor = o.batch(T, P, Y, threads=8)
+    out = run(M.chemistry(), "mixed", T, P, Y, gpu)
+    assert np.all(np.isfinite(out))
+    Ka = mech["Ka"]
+    sfloor = state_floor(floor, Ka)
+    err = relative_error(out, ref, sfloor)
+    below = np.mean(np.maximum(np.abs(ref[:Ka]), floor[:Ka]) < sfloor[:Ka])
+    print(name, "mixed ω max/median", err[:Ka].max(), np.median(err[:Ka]), "λ", err[Ka].max(), "μ", err[Ka + 1].max(),
+          "D", err[Ka + 2:].max(), "fraction of ω under the state floor", below, flush=True)
+    assert err[:Ka].max() <= MIXED_RATES_TOL
+    assert err[Ka:].max() <= MIXED_TRANSPORT_TOL
+
+
+@pytest.mark.gpu
+def test_mixed_lanes_agree_on_repeated_state(gpu):
+    M, mech, o = setup("gri-53")
+    T, P, Y = synth.config_states("gri-53", 1, 4)
+    n = 4096
+    out = run(M.chemistry(), "mixed", np.repeat(T, n), np.repeat(P, n), np.repeat(Y, n, axis=1), gpu)
+    assert np.all(out == out[:, :1])
