@@ -12,6 +12,7 @@
 #include "emit.hpp"
 #include "forms.hpp"
 #include "mechanism.hpp"
+#include "mlp.hpp"
 #include "yaml.hpp"
 
 using namespace combustion;
@@ -30,6 +31,10 @@ struct cb_program {
 
 struct cb_module {
   std::unique_ptr<Form> form;
+};
+
+struct cb_mlp {
+  std::unique_ptr<Mlp> net;
 };
 
 namespace {
@@ -356,5 +361,110 @@ int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, s
 }
 
 void cb_module_free(cb_module* mod) { delete mod; }
+
+int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w, cb_module** out) {
+  return guard([&] {
+    require(p && w && out && w->params, "null argument");
+    require(p->kind == ProgramKind::Chemistry, "the surrogate form needs a program built by cb_build_chemistry");
+    const int NF = (int)p->mechanism->species.len() + 2;
+    const MlpLayout lay(NF, w->width, w->depth, NF);
+    SurrogateWeights sw;
+    sw.width = w->width;
+    sw.depth = w->depth;
+    sw.params.assign(w->params, w->params + lay.n_params);
+    auto copy = [&](std::vector<float>& v, const float* src) {
+      require(src, "null normalisation array");
+      v.assign(src, src + NF);
+    };
+    copy(sw.f_mean, w->feature_mean);
+    copy(sw.f_rstd, w->feature_rstd);
+    copy(sw.lo, w->lower);
+    copy(sw.hi, w->upper);
+    copy(sw.o_mean, w->output_mean);
+    copy(sw.o_std, w->output_std);
+    ProgramView v{p->mechanism, &p->function, &p->linear, p->kind, &p->chemistry};
+    auto* m = new cb_module;
+    try {
+      m->form = assemble_surrogate(v, sw);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int cb_module_fallback_count(cb_module* mod, void* stream, uint32_t* count) {
+  return guard([&] {
+    require(mod && mod->form && count, "null argument");
+    *count = surrogate_fallback_count(*mod->form, (hipStream_t)stream);
+  });
+}
+
+int cb_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, cb_mlp** out) {
+  return guard([&] {
+    require(out, "null argument");
+    auto* m = new cb_mlp;
+    try {
+      m->net = std::make_unique<Mlp>(n_in, width, depth, n_out);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+void cb_mlp_free(cb_mlp* mlp) { delete mlp; }
+
+int cb_mlp_sizes(const cb_mlp* mlp, int64_t* n_params, int32_t* in_pad, int32_t* out_pad, int64_t* layer_offsets) {
+  return guard([&] {
+    require(mlp && mlp->net, "null network");
+    if (n_params) *n_params = mlp->net->n_params();
+    if (in_pad) *in_pad = (int32_t)mlp->net->in_pad();
+    if (out_pad) *out_pad = (int32_t)mlp->net->out_pad();
+    if (layer_offsets)
+      for (size_t l = 0; l < mlp->net->layers().size(); l++) {
+        layer_offsets[2 * l] = mlp->net->layers()[l].w;
+        layer_offsets[2 * l + 1] = mlp->net->layers()[l].b;
+      }
+  });
+}
+
+int cb_mlp_workspace_bytes(const cb_mlp* mlp, int64_t n, size_t* bytes) {
+  return guard([&] {
+    require(mlp && mlp->net && bytes && n >= 0, "bad argument");
+    *bytes = mlp->net->workspace_bytes(n);
+  });
+}
+
+int cb_mlp_forward(const cb_mlp* mlp, const float* params, const uint16_t* params_bf16, const float* x, int64_t n, float* y,
+                   void* workspace, void* stream) {
+  return guard([&] {
+    require(mlp && mlp->net, "null network");
+    require(n == 0 || (params && params_bf16 && x && y && workspace), "null argument");
+    require(n < (int64_t(1) << 31), "n must be below 2^31");
+    mlp->net->forward(params, params_bf16, x, n, y, workspace, (hipStream_t)stream);
+  });
+}
+
+int cb_mlp_backward_mse(const cb_mlp* mlp, const uint16_t* params_bf16, const float* y, const float* target, int64_t n,
+                        void* workspace, float* grad, void* stream) {
+  return guard([&] {
+    require(mlp && mlp->net, "null network");
+    require(n == 0 || (params_bf16 && y && target && workspace && grad), "null argument");
+    require(n < (int64_t(1) << 31), "n must be below 2^31");
+    mlp->net->backward_mse(params_bf16, y, target, n, workspace, grad, (hipStream_t)stream);
+  });
+}
+
+int cb_mlp_adam(const cb_mlp* mlp, float* params, uint16_t* params_bf16, const float* grad, float* m1, float* m2, float lr,
+                float beta1, float beta2, float eps, int32_t step, void* stream) {
+  return guard([&] {
+    require(mlp && mlp->net && params && params_bf16 && grad && m1 && m2, "null argument");
+    require(step >= 1, "Adam steps count from 1");
+    mlp->net->adam(params, params_bf16, grad, m1, m2, lr, beta1, beta2, eps, step, (hipStream_t)stream);
+  });
+}
 
 }  // extern "C"

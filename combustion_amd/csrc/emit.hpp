@@ -9,7 +9,6 @@ namespace combustion {
 struct StraightOptions {
   unsigned block_size = 256;
   bool reciprocal = false;   // a / b as a * (1/b), 1/b by v_rcp_f64 and two Newton steps (within 1 ulp of 1/b)
-  unsigned waves_per_eu = 0; // occupancy hint (0: compiler default)
   bool f32 = false;          // evaluate in FP32 (inputs and outputs stay FP64 arrays); exp/log/rcp by the hardware ops
 };
 // Straight-line kernel: signature (const double* const* in, double* const* out, unsigned n, unsigned offset).
@@ -17,5 +16,11 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
 
 // Mixed-precision rates kernel template (kernels/rates_mixed.hip), embedded at build time.
 extern const char* const rates_mixed_source;
+// Mixture-averaged transport kernel template (kernels/transport_tiled.hip).
+extern const char* const transport_tiled_source;
+// Surrogate network kernels (kernels/mlp.hip).
+extern const char* const mlp_source;
+// Surrogate inference kernel (kernels/surrogate.hip).
+extern const char* const surrogate_source;
 
 }  // namespace combustion

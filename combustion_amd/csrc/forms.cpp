@@ -1,6 +1,5 @@
 #include "forms.hpp"
 
-#include <cstdlib>
 #include <vector>
 
 #include "device.hpp"
@@ -45,19 +44,12 @@ class Straight : public SoAForm {
 StraightOptions straight_options(int form, unsigned block_size) {
   StraightOptions o;
   o.block_size = block_size ? block_size : 256;
-  if (form == 1) {
-    o.reciprocal = true;
-    if (const char* w = std::getenv("COMBUSTION_AMD_WAVES_PER_EU")) o.waves_per_eu = (unsigned)std::atoi(w);
-  }
+  o.reciprocal = form == 1;
   return o;
 }
 
-bool use_tiled(const ProgramView& p, int form) {
-  if (form == 2) return true;
-  if (form != 1) return false;
-  if (const char* t = std::getenv("COMBUSTION_AMD_FP64_TILED")) return std::atoi(t) != 0;
-  return false;
-}
+// Chemistry programs (without the reactor right-hand side) take the tiled kernels in forms 1 and 2.
+bool use_tiled(const ProgramView& p, int form) { return form == 2 || (form == 1 && tiled_applies(p)); }
 
 std::string form_source(const ProgramView& p, int form, unsigned block_size) {
   if (form < 0 || form > 2) throw ArgumentError("unknown form");

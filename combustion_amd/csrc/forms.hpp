@@ -56,8 +56,18 @@ void compile_form(const ProgramView& p, int form, unsigned block_size);
 std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size);
 
 // Tiled forms (tiled.cpp).
+bool tiled_applies(const ProgramView& p);
 std::string tiled_source(const ProgramView& p, bool mixed, unsigned block_size);
 void compile_tiled(const ProgramView& p, bool mixed, unsigned block_size);
 std::unique_ptr<Form> assemble_tiled(const ProgramView& p, bool mixed, unsigned block_size);
+
+// Transport surrogate (kernels/surrogate.hip): network parameters in the Mlp layout for n_in = n_out = K+2 and
+// the per-feature and per-output normalisation, training box included.
+struct SurrogateWeights {
+  int width = 256, depth = 3;
+  std::vector<float> params, f_mean, f_rstd, lo, hi, o_mean, o_std;
+};
+std::unique_ptr<Form> assemble_surrogate(const ProgramView& p, const SurrogateWeights& w);
+unsigned surrogate_fallback_count(Form& f, hipStream_t stream);
 
 }  // namespace combustion

@@ -35,7 +35,6 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
          "__device__ __forceinline__ float exp32(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }\n"
          "__device__ __forceinline__ float log32(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }\n";
   s << "extern \"C\" __global__ void __launch_bounds__(" << o.block_size << ") ";
-  if (o.waves_per_eu) s << "__attribute__((amdgpu_waves_per_eu(" << o.waves_per_eu << ", " << o.waves_per_eu << "))) ";
   s << name
     << "(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n, unsigned offset) {\n"
     << "  const unsigned id = offset + blockIdx.x * blockDim.x + threadIdx.x;\n"

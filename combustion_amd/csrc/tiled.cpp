@@ -1,6 +1,9 @@
-// Mechanism-tiled forms. The mixed-precision form launches two kernels on the caller's stream:
-//   rates_mixed (kernels/rates_mixed.hip): reaction tiles on BF16 MFMA, mechanism tables in device memory;
-//   the transport section of the program emitted as straight-line FP32 code.
+// Mechanism-tiled forms, each launching its kernels on the caller's stream:
+//   fp64  (chemistry programs): the rates section as straight-line FP64 code, then transport_tiled in FP64;
+//   mixed: rates_mixed (reaction tiles on BF16 MFMA, mechanism tables in device memory), then transport_tiled
+//          in FP32.
+// transport_tiled (kernels/transport_tiled.hip) evaluates the Wilke sums as three K×K MFMA contractions and the
+// binary-diffusion polynomials as (pairs × D)·(D × states) MFMA tiles.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -9,6 +12,7 @@
 #include "device.hpp"
 #include "emit.hpp"
 #include "forms.hpp"
+#include "mlp.hpp"
 #include "soa.hpp"
 
 namespace combustion {
@@ -183,86 +187,172 @@ std::string rates_mixed_text(const MixedTables& t) {
   return s.str();
 }
 
-ast::Linear transport_program(const ProgramView& p) {
-  ChemistryOptions o;
-  o.rates = false;
-  o.transport = true;
-  return ast::linearize(chemistry(*p.mechanism, p.chemistry->polynomials.get(), p.chemistry->scales, o));
+// Tables of kernels/transport_tiled.hip: unscaled fits (transport.rs:125-136) and the Wilke and diffusion
+// operands in the lane layouts of v_mfma_{f64,f32}_16x16x4 (A[row L%16][col L/16]).
+struct TransportTables {
+  int K, KP, DK, NTILE;
+  double T0;
+  std::vector<double> rcp_W, W, fit_c, fit_v, wilke, dcoef;
+  std::vector<int32_t> tiles;
+};
+
+TransportTables transport_tables(const Mechanism& m, const Polynomials& p, double T0) {
+  TransportTables t;
+  const Species& sp = m.species;
+  t.K = (int)sp.len();
+  t.KP = (t.K + 15) / 16 * 16;
+  t.DK = (p.D + 3) / 4 * 4;
+  if (t.DK > 8) throw Unsupported("transport fits longer than 8 coefficients");
+  t.T0 = T0;
+  const int K = t.K, KP = t.KP, D = p.D, DK = t.DK;
+  for (int k = 0; k < K; k++) {
+    t.W.push_back(sp.molar_mass[k]);
+    t.rcp_W.push_back(1. / sp.molar_mass[k]);
+    for (int d = 0; d < DK; d++) {
+      t.fit_c.push_back(d < D ? p.conductivityIVT[k * D + d] : 0.);
+      t.fit_v.push_back(d < D ? p.VviscosityIVVT[k * D + d] : 0.);
+    }
+  }
+  // Wilke (transport.rs:148-158): Φ_kj = (a_kj + b_kj v_k/v_j)², a = (8 (1 + W_k/W_j))^-1/4, b = a (W_j/W_k)^1/4,
+  // stored as [row tile mt][column block kk][a², a b, b²][lane].
+  t.wilke.assign((size_t)(KP / 16) * (KP / 4) * 3 * 64, 0.);
+  for (int mt = 0; mt < KP / 16; mt++)
+    for (int kk = 0; kk < KP / 4; kk++)
+      for (int L = 0; L < 64; L++) {
+        const int k = 16 * mt + L % 16, j = 4 * kk + L / 16;
+        if (k >= K || j >= K) continue;
+        const double W_k = sp.molar_mass[k], W_j = sp.molar_mass[j];
+        const double a = std::sqrt(1. / std::sqrt(8.) * 1. / std::sqrt(1. + W_k / W_j));
+        const double b = a * std::sqrt(std::sqrt(W_j / W_k));
+        const size_t at = (((size_t)mt * (KP / 4) + kk) * 3) * 64 + L;
+        t.wilke[at] = a * a;
+        t.wilke[at + 64] = a * b;
+        t.wilke[at + 128] = b * b;
+      }
+  // Diffusion tiles: 16 consecutive j < k for one k; A[m][d] = coefficient d of d_{k, j0+m} (zero past j = k-1).
+  for (int k = 1; k < K; k++)
+    for (int j0 = 0; j0 < k; j0 += 16) {
+      t.tiles.push_back(k);
+      t.tiles.push_back(j0);
+      for (int blk = 0; blk < DK / 4; blk++)
+        for (int L = 0; L < 64; L++) {
+          const int j = j0 + L % 16, d = 4 * blk + L / 16;
+          t.dcoef.push_back(j < k && d < D ? p.diffusivityITVT[((size_t)k * K + j) * D + d] : 0.);
+        }
+    }
+  t.NTILE = (int)t.tiles.size() / 2;
+  return t;
 }
 
-StraightOptions f32_options(unsigned block_size) {
-  StraightOptions o;
-  o.block_size = block_size ? block_size : 256;
-  o.f32 = true;
-  return o;
+std::string transport_text(const TransportTables& t, bool f64) {
+  std::ostringstream s;
+  s << "#define K " << t.K << "\n#define KP " << t.KP << "\n#define NTILE " << t.NTILE << "\n#define DK " << t.DK << "\n#define F64 "
+    << (f64 ? 1 : 0) << "\n"
+    << transport_tiled_source;
+  return s.str();
 }
+
+std::vector<float> to_f32(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+const Polynomials& polynomials(const ProgramView& p) {
+  if (!p.chemistry->polynomials) throw Unsupported("the program has no transport fits");
+  return *p.chemistry->polynomials;
+}
+
+// transport_tiled with its tables in device memory; writes [λ, μ, D_0..D_{K-1}] to out[0..K+2).
+class TransportTiled {
+ public:
+  TransportTiled(const ProgramView& p, bool f64) : t_(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0)), f64_(f64) {
+    kernel_.load(device::compile(transport_text(t_, f64), "transport_tiled", hip_options(true)), "transport_tiled");
+    d_rcp_W_ = upload(t_.rcp_W);
+    d_W_ = upload(t_.W);
+    d_tiles_ = upload(t_.tiles);
+    if (f64) {
+      d_c_ = upload(t_.fit_c);
+      d_v_ = upload(t_.fit_v);
+      d_wilke_ = upload(t_.wilke);
+      d_dcoef_ = upload(t_.dcoef);
+    } else {
+      d_c_ = upload(to_f32(t_.fit_c));
+      d_v_ = upload(to_f32(t_.fit_v));
+      d_wilke_ = upload(to_f32(t_.wilke));
+      d_dcoef_ = upload(to_f32(t_.dcoef));
+    }
+  }
+  ~TransportTiled() {
+    for (void* ptr : {(void*)d_rcp_W_, (void*)d_W_, (void*)d_tiles_, d_c_, d_v_, d_wilke_, d_dcoef_})
+      if (ptr) (void)hipFree(ptr);
+  }
+  // Every state, or (with idx and count) the count states listed in idx.
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream, const unsigned* idx = nullptr,
+              const unsigned* count = nullptr) {
+    unsigned n32 = (unsigned)n;
+    double T0 = t_.T0;
+    void* args[] = {&in, &out, &n32, &d_rcp_W_, &d_W_, &d_c_, &d_v_, &d_wilke_, &d_dcoef_, &d_tiles_, &T0, &idx, &count};
+    const unsigned states_per_block = 16;  // 4 waves share 16 states
+    unsigned grid = (unsigned)((n + states_per_block - 1) / states_per_block);
+    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch transport_tiled");
+  }
+
+ private:
+  TransportTables t_;
+  bool f64_;
+  device::Kernel kernel_;
+  double *d_rcp_W_ = nullptr, *d_W_ = nullptr;
+  int32_t* d_tiles_ = nullptr;
+  void *d_c_ = nullptr, *d_v_ = nullptr, *d_wilke_ = nullptr, *d_dcoef_ = nullptr;
+};
 
 void check_program(const ProgramView& p) {
   if (p.kind != ProgramKind::Chemistry) throw Unsupported("tiled forms need a program built by cb_build_chemistry");
   if (p.chemistry->options.rhs) throw Unsupported("tiled forms do not evaluate the reactor right-hand side");
 }
 
-class Mixed : public SoAForm {
+// The rates section of a chemistry program as straight-line FP64 code (contraction and reciprocal division).
+std::string rates_fp64_text(const ProgramView& p) {
+  ChemistryOptions o;
+  o.transport = false;
+  StraightOptions so;
+  so.reciprocal = true;
+  return emit_straight(ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o)), "rates_fp64", so);
+}
+
+// rates_mixed with its tables in device memory; writes ω to out[0..Ka).
+class RatesMixed {
  public:
-  Mixed(const ProgramView& p, unsigned block_size) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
-    check_program(p);
-    const auto& o = p.chemistry->options;
-    if (o.rates) {
-      t_ = mixed_tables(*p.mechanism);
-      rates_.load(device::compile(rates_mixed_text(t_), "rates_mixed", hip_options(true)), "rates_mixed");
-      d_fwd_ = upload(t_.A_fwd);
-      d_rev_ = upload(t_.A_rev);
-      d_nu_ = upload(t_.A_nu);
-      d_rate_ = upload(t_.rate);
-      d_k0_ = upload(t_.k0);
-      d_troe_ = upload(t_.troe);
-      d_info_ = upload(t_.slots);
-      d_tile_ = upload(t_.tile_info);
-      d_eff_ = upload(t_.eff);
-      d_rcp_W_ = upload(t_.rcp_W);
-      d_gibbs_ = upload(t_.gibbs);
-      d_split_ = upload(t_.split);
-      n_rates_ = t_.KA;
-    }
-    if (o.transport) {
-      const StraightOptions so = f32_options(block_size);
-      transport_block_ = so.block_size;
-      transport_.load(device::compile(emit_straight(transport_program(p), "transport_f32", so), "transport_f32", hip_options(true)), "transport_f32");
-      has_transport_ = true;
-    }
+  explicit RatesMixed(const Mechanism& m) : t_(mixed_tables(m)) {
+    kernel_.load(device::compile(rates_mixed_text(t_), "rates_mixed", hip_options(true)), "rates_mixed");
+    d_fwd_ = upload(t_.A_fwd);
+    d_rev_ = upload(t_.A_rev);
+    d_nu_ = upload(t_.A_nu);
+    d_rate_ = upload(t_.rate);
+    d_k0_ = upload(t_.k0);
+    d_troe_ = upload(t_.troe);
+    d_info_ = upload(t_.slots);
+    d_tile_ = upload(t_.tile_info);
+    d_eff_ = upload(t_.eff);
+    d_rcp_W_ = upload(t_.rcp_W);
+    d_gibbs_ = upload(t_.gibbs);
+    d_split_ = upload(t_.split);
   }
-  ~Mixed() override {
+  ~RatesMixed() {
     for (void* ptr : {(void*)d_fwd_, (void*)d_rev_, (void*)d_nu_, (void*)d_rate_, (void*)d_k0_, (void*)d_troe_, (void*)d_info_,
                       (void*)d_tile_, (void*)d_eff_, (void*)d_rcp_W_, (void*)d_gibbs_, (void*)d_split_})
       if (ptr) (void)hipFree(ptr);
   }
-
- protected:
-  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
     unsigned n32 = (unsigned)n;
-    if (n_rates_) {
-      const unsigned states_per_block = 16 * 4;
-      unsigned grid = (unsigned)((n + states_per_block - 1) / states_per_block);
-      double pos = t_.ln_clamp_pos, neg = t_.ln_clamp_neg;
-      void* args[] = {&in, &out, &n32, &d_fwd_, &d_rev_, &d_nu_, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_tile_, &d_eff_,
-                      &d_rcp_W_, &d_gibbs_, &d_split_, &pos, &neg};
-      device::check(hipModuleLaunchKernel(rates_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch rates_mixed");
-    }
-    if (has_transport_) {
-      double* const* tout = out + n_rates_;
-      unsigned offset = 0;
-      void* args[] = {&in, &tout, &n32, &offset};
-      unsigned grid = (unsigned)((n + transport_block_ - 1) / transport_block_);
-      device::check(hipModuleLaunchKernel(transport_.function, grid, 1, 1, transport_block_, 1, 1, 0, stream, args, nullptr), "launch transport_f32");
-    }
+    const unsigned states_per_block = 16 * 4;
+    unsigned grid = (unsigned)((n + states_per_block - 1) / states_per_block);
+    double pos = t_.ln_clamp_pos, neg = t_.ln_clamp_neg;
+    void* args[] = {&in, &out, &n32, &d_fwd_, &d_rev_, &d_nu_, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_tile_, &d_eff_,
+                    &d_rcp_W_, &d_gibbs_, &d_split_, &pos, &neg};
+    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch rates_mixed");
   }
 
  private:
   MixedTables t_;
-  device::Kernel rates_, transport_;
-  int n_rates_ = 0;
-  bool has_transport_ = false;
-  unsigned transport_block_ = 256;
+  device::Kernel kernel_;
   uint16_t *d_fwd_ = nullptr, *d_rev_ = nullptr, *d_nu_ = nullptr;
   double *d_rate_ = nullptr, *d_k0_ = nullptr;
   float *d_troe_ = nullptr, *d_eff_ = nullptr;
@@ -270,28 +360,203 @@ class Mixed : public SoAForm {
   double *d_rcp_W_ = nullptr, *d_gibbs_ = nullptr, *d_split_ = nullptr;
 };
 
+// The rates section as straight-line FP64 code; writes ω to out[0..Ka).
+class RatesFp64 {
+ public:
+  explicit RatesFp64(const ProgramView& p) {
+    kernel_.load(device::compile(rates_fp64_text(p), "rates_fp64", hip_options(true)), "rates_fp64");
+  }
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
+    unsigned n32 = (unsigned)n, offset = 0, block = 256;
+    void* args[] = {&in, &out, &n32, &offset};
+    unsigned grid = (unsigned)((n + block - 1) / block);
+    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, block, 1, 1, 0, stream, args, nullptr), "launch rates_fp64");
+  }
+
+ private:
+  device::Kernel kernel_;
+};
+
+class Tiled : public SoAForm {
+ public:
+  Tiled(const ProgramView& p, bool mixed) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
+    check_program(p);
+    const auto& o = p.chemistry->options;
+    if (o.rates) {
+      n_rates_ = (int)p.mechanism->active;
+      if (mixed)
+        mixed_ = std::make_unique<RatesMixed>(*p.mechanism);
+      else
+        fp64_ = std::make_unique<RatesFp64>(p);
+    }
+    if (o.transport) transport_ = std::make_unique<TransportTiled>(p, !mixed);
+  }
+
+ protected:
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
+    if (mixed_) mixed_->launch(in, out, n, stream);
+    if (fp64_) fp64_->launch(in, out, n, stream);
+    if (transport_) transport_->launch(in, out + n_rates_, n, stream);
+  }
+
+ private:
+  std::unique_ptr<RatesMixed> mixed_;
+  std::unique_ptr<RatesFp64> fp64_;
+  std::unique_ptr<TransportTiled> transport_;
+  int n_rates_ = 0;
+};
+
+std::vector<uint16_t> to_bf16(const std::vector<float>& v) {
+  std::vector<uint16_t> out;
+  for (float x : v) out.push_back(bf16_bits(x));
+  return out;
+}
+
+std::string surrogate_text(int K, const MlpLayout& lay, int DK) {
+  std::ostringstream s;
+  s << "#define K " << K << "\n#define IN " << lay.in_pad << "\n#define WIDTH " << lay.width << "\n#define DEPTH " << lay.depth
+    << "\n#define OUT " << lay.out_pad << "\n#define DK " << DK << "\n"
+    << surrogate_source;
+  return s.str();
+}
+
+// Mixed rates, then the transport surrogate with the FP32 transport kernel on the states outside its box.
+class Surrogate : public SoAForm {
+ public:
+  Surrogate(const ProgramView& p, const SurrogateWeights& w) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
+    check_program(p);
+    const auto& o = p.chemistry->options;
+    if (!o.transport) throw Unsupported("the surrogate form needs the transport section");
+    const Mechanism& m = *p.mechanism;
+    const int K = (int)m.species.len(), NF = K + 2;
+    const MlpLayout lay(NF, w.width, w.depth, NF);
+    if ((long)w.params.size() != lay.n_params) throw ArgumentError("surrogate parameters do not match the network size");
+    for (const auto* v : {&w.f_mean, &w.f_rstd, &w.lo, &w.hi, &w.o_mean, &w.o_std})
+      if ((int)v->size() != NF) throw ArgumentError("surrogate normalisation arrays need K+2 entries");
+    if (o.rates) {
+      n_rates_ = (int)m.active;
+      rates_ = std::make_unique<RatesMixed>(m);
+    }
+    transport_ = std::make_unique<TransportTiled>(p, false);
+    const Polynomials& poly = polynomials(p);
+    const int D = poly.D, DK = D;
+    if (D > 8) throw Unsupported("transport fits longer than 8 coefficients");
+    // Baselines: the bulk species' conductivity and viscosity fits, and d_kb against the bulk species (species 0 for
+    // the bulk species itself).
+    const int b = K - 1;
+    std::vector<float> bc, bv, bd;
+    for (int d = 0; d < D; d++) {
+      bc.push_back((float)poly.conductivityIVT[b * D + d]);
+      bv.push_back((float)poly.VviscosityIVVT[b * D + d]);
+    }
+    for (int k = 0; k < K; k++) {
+      const int j = k == b ? 0 : b;
+      for (int d = 0; d < D; d++) bd.push_back((float)poly.diffusivityITVT[((size_t)k * K + j) * D + d]);
+    }
+    std::vector<long> offsets;
+    for (const auto& L : lay.layers) {
+      offsets.push_back(L.w);
+      offsets.push_back(L.b);
+    }
+    kernel_.load(device::compile(surrogate_text(K, lay, DK), "surrogate_transport", hip_options(true)), "surrogate_transport");
+    d_w16_ = upload(to_bf16(w.params));
+    d_params_ = upload(w.params);
+    d_offsets_ = upload(offsets);
+    d_f_mean_ = upload(w.f_mean);
+    d_f_rstd_ = upload(w.f_rstd);
+    d_lo_ = upload(w.lo);
+    d_hi_ = upload(w.hi);
+    d_o_mean_ = upload(w.o_mean);
+    d_o_std_ = upload(w.o_std);
+    d_bc_ = upload(bc);
+    d_bv_ = upload(bv);
+    d_bd_ = upload(bd);
+    T0_ = (float)p.chemistry->scales.T0;
+    device::check(hipMalloc(&d_count_, sizeof(unsigned)), "hipMalloc");
+  }
+  ~Surrogate() override {
+    for (void* ptr : {(void*)d_w16_, (void*)d_params_, (void*)d_offsets_, (void*)d_f_mean_, (void*)d_f_rstd_, (void*)d_lo_,
+                      (void*)d_hi_, (void*)d_o_mean_, (void*)d_o_std_, (void*)d_bc_, (void*)d_bv_, (void*)d_bd_, (void*)d_count_,
+                      (void*)d_fallback_})
+      if (ptr) (void)hipFree(ptr);
+  }
+  // States the last call sent to the plain transport kernel (waits for the stream).
+  unsigned fallback_count(hipStream_t stream) {
+    unsigned c = 0;
+    device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    device::check(hipMemcpy(&c, d_count_, sizeof c, hipMemcpyDeviceToHost), "hipMemcpy");
+    return c;
+  }
+
+ protected:
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
+    if (n > capacity_) {
+      if (d_fallback_) device::check(hipFree(d_fallback_), "hipFree");
+      d_fallback_ = nullptr;
+      device::check(hipMalloc(&d_fallback_, n * sizeof(unsigned)), "hipMalloc");
+      capacity_ = n;
+    }
+    if (rates_) rates_->launch(in, out, n, stream);
+    double* const* tout = out + n_rates_;
+    device::check(hipMemsetAsync(d_count_, 0, sizeof(unsigned), stream), "hipMemsetAsync");
+    unsigned n32 = (unsigned)n;
+    void* args[] = {&in, &tout, &n32, &d_w16_, &d_params_, &d_offsets_, &d_f_mean_, &d_f_rstd_, &d_lo_, &d_hi_,
+                    &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_bd_, &T0_, &d_fallback_, &d_count_};
+    const unsigned per_block = 128;
+    device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + per_block - 1) / per_block), 1, 1, 256, 1, 1, 0, stream, args,
+                                        nullptr),
+                  "launch surrogate_transport");
+    transport_->launch(in, tout, n, stream, d_fallback_, d_count_);
+  }
+
+ private:
+  std::unique_ptr<RatesMixed> rates_;
+  std::unique_ptr<TransportTiled> transport_;
+  int n_rates_ = 0;
+  device::Kernel kernel_;
+  uint16_t* d_w16_ = nullptr;
+  float *d_params_ = nullptr, *d_f_mean_ = nullptr, *d_f_rstd_ = nullptr, *d_lo_ = nullptr, *d_hi_ = nullptr, *d_o_mean_ = nullptr,
+        *d_o_std_ = nullptr, *d_bc_ = nullptr, *d_bv_ = nullptr, *d_bd_ = nullptr;
+  long* d_offsets_ = nullptr;
+  float T0_ = 1000.f;
+  unsigned *d_count_ = nullptr, *d_fallback_ = nullptr;
+  size_t capacity_ = 0;
+};
+
 }  // namespace
 
-std::string tiled_source(const ProgramView& p, bool mixed, unsigned block_size) {
+bool tiled_applies(const ProgramView& p) { return p.kind == ProgramKind::Chemistry && !p.chemistry->options.rhs; }
+
+std::string tiled_source(const ProgramView& p, bool mixed, unsigned) {
   check_program(p);
-  if (!mixed) throw Unsupported("the tiled FP64 form is not built yet");
   std::string s;
-  if (p.chemistry->options.rates) s += rates_mixed_text(mixed_tables(*p.mechanism));
-  if (p.chemistry->options.transport) s += "\n// ---- transport (FP32 straight-line) ----\n" + emit_straight(transport_program(p), "transport_f32", f32_options(block_size));
+  if (p.chemistry->options.rates) s += mixed ? rates_mixed_text(mixed_tables(*p.mechanism)) : rates_fp64_text(p);
+  if (p.chemistry->options.transport)
+    s += "\n// ---- transport ----\n" + transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed);
   return s;
 }
 
-void compile_tiled(const ProgramView& p, bool mixed, unsigned block_size) {
+void compile_tiled(const ProgramView& p, bool mixed, unsigned) {
   check_program(p);
-  if (!mixed) throw Unsupported("the tiled FP64 form is not built yet");
-  if (p.chemistry->options.rates) device::compile(rates_mixed_text(mixed_tables(*p.mechanism)), "rates_mixed", hip_options(true));
+  if (p.chemistry->options.rates) {
+    if (mixed)
+      device::compile(rates_mixed_text(mixed_tables(*p.mechanism)), "rates_mixed", hip_options(true));
+    else
+      device::compile(rates_fp64_text(p), "rates_fp64", hip_options(true));
+  }
   if (p.chemistry->options.transport)
-    device::compile(emit_straight(transport_program(p), "transport_f32", f32_options(block_size)), "transport_f32", hip_options(true));
+    device::compile(transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed), "transport_tiled",
+                    hip_options(true));
 }
 
-std::unique_ptr<Form> assemble_tiled(const ProgramView& p, bool mixed, unsigned block_size) {
-  if (!mixed) throw Unsupported("the tiled FP64 form is not built yet");
-  return std::make_unique<Mixed>(p, block_size);
+std::unique_ptr<Form> assemble_tiled(const ProgramView& p, bool mixed, unsigned) { return std::make_unique<Tiled>(p, mixed); }
+
+std::unique_ptr<Form> assemble_surrogate(const ProgramView& p, const SurrogateWeights& w) { return std::make_unique<Surrogate>(p, w); }
+
+unsigned surrogate_fallback_count(Form& f, hipStream_t stream) {
+  auto* s = dynamic_cast<Surrogate*>(&f);
+  if (!s) throw ArgumentError("not a surrogate module");
+  return s->fallback_count(stream);
 }
 
 }  // namespace combustion

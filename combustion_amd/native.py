@@ -65,6 +65,15 @@ SIGNATURES = {
     "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
     "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
     "cb_module_free": (None, [_P]),
+    "cb_hip_assemble_surrogate": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "cb_module_fallback_count": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    "cb_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "cb_mlp_free": (None, [_P]),
+    "cb_mlp_sizes": (C.c_int, [_P, _I64P, _I32P, _I32P, _I64P]),
+    "cb_mlp_workspace_bytes": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_size_t)]),
+    "cb_mlp_forward": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P]),
+    "cb_mlp_backward_mse": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P]),
+    "cb_mlp_adam": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, _P]),
 }
 
 
@@ -280,3 +289,59 @@ class Module:
         if time_kernel:
             self.last_kernel_ms = ms.value
         return outputs
+
+
+class SurrogateWeights(C.Structure):
+    """cb_surrogate_weights."""
+    _fields_ = [("width", C.c_int32), ("depth", C.c_int32)] + [
+        (name, C.POINTER(C.c_float)) for name in ("params", "feature_mean", "feature_rstd", "lower", "upper", "output_mean", "output_std")]
+
+
+def _tensor(x, dtype):
+    if not getattr(x, "is_cuda", False) or str(x.dtype) != dtype or not x.is_contiguous():
+        raise ValueError(f"expected a contiguous {dtype} device tensor")
+    return x.data_ptr()
+
+
+class Network:
+    """The transport surrogate's MLP kernels (cb_mlp_*): n_in -> width (SiLU) x depth -> n_out, BF16 weights,
+    FP32 accumulation. Parameters, activations and gradients are torch device tensors owned by the caller."""
+
+    def __init__(self, n_in, width, depth, n_out):
+        h = _P()
+        _check(lib().cb_mlp_create(n_in, width, depth, n_out, C.byref(h)))
+        self._h = h
+        self.n_in, self.width, self.depth, self.n_out = n_in, width, depth, n_out
+        n_params, in_pad, out_pad = C.c_int64(), C.c_int32(), C.c_int32()
+        offsets = (C.c_int64 * (2 * (depth + 1)))()
+        _check(lib().cb_mlp_sizes(h, C.byref(n_params), C.byref(in_pad), C.byref(out_pad), offsets))
+        self.n_params, self.in_pad, self.out_pad = n_params.value, in_pad.value, out_pad.value
+        dims = [self.in_pad] + [width] * depth + [self.out_pad]
+        # (W offset, b offset, out, in) per layer
+        self.layers = [(offsets[2 * l], offsets[2 * l + 1], dims[l + 1], dims[l]) for l in range(depth + 1)]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cb_mlp_free(self._h)
+            self._h = None
+
+    def workspace_bytes(self, n):
+        b = C.c_size_t()
+        _check(lib().cb_mlp_workspace_bytes(self._h, n, C.byref(b)))
+        return b.value
+
+    def forward(self, params, params16, x, y, workspace, stream=None):
+        n = x.shape[0]
+        _check(lib().cb_mlp_forward(self._h, _tensor(params, "torch.float32"), _tensor(params16, "torch.bfloat16"),
+                                    _tensor(x, "torch.float32"), n, _tensor(y, "torch.float32"), _tensor(workspace, "torch.uint8"), stream))
+
+    def backward_mse(self, params16, y, target, workspace, grad, stream=None):
+        n = y.shape[0]
+        _check(lib().cb_mlp_backward_mse(self._h, _tensor(params16, "torch.bfloat16"), _tensor(y, "torch.float32"),
+                                         _tensor(target, "torch.float32"), n, _tensor(workspace, "torch.uint8"),
+                                         _tensor(grad, "torch.float32"), stream))
+
+    def adam(self, params, params16, grad, m1, m2, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        _check(lib().cb_mlp_adam(self._h, _tensor(params, "torch.float32"), _tensor(params16, "torch.bfloat16"),
+                                 _tensor(grad, "torch.float32"), _tensor(m1, "torch.float32"), _tensor(m2, "torch.float32"),
+                                 lr, beta1, beta2, eps, step, stream))

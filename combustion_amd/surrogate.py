@@ -1,0 +1,200 @@
+"""Learned mixture-averaged transport: training data from reactor trajectories, training with the project's
+MLP kernels (kernels/mlp.hip through cb_mlp_*), and the surrogate form (cb_hip_assemble_surrogate).
+
+The network maps the state's features f = (T, ln P, Y_0..Y_{K-1}) to r_o = ln(p_o / b_o) for the K+2 transport
+outputs p = [λ, μ, D_0..D_{K-1}] of the chemistry program (transport.rs:141-177), relative to a baseline b that
+costs O(K) per state: the bulk species' own conductivity c_b(ℓ)√T and viscosity v_b(ℓ)²√T, and the binary
+diffusion coefficient of each species against the bulk species d_kb(ℓ) T^1.5/P (against species 0 for the bulk
+species), all from the transport fits in ℓ = ln(T/T0) (transport.rs:125-136). Features and targets are
+standardised with the training set's means and deviations; the box [min, max] of the training features is the
+domain where the network is used — states outside it are evaluated by the plain transport kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import native
+
+
+class Trained:
+    """A trained surrogate: network parameters (flat FP32, cb_mlp layout) and normalisation."""
+
+    FIELDS = ("params", "f_mean", "f_rstd", "lo", "hi", "o_mean", "o_std")
+
+    def __init__(self, width, depth, params, f_mean, f_rstd, lo, hi, o_mean, o_std):
+        self.width, self.depth = int(width), int(depth)
+        self.params, self.f_mean, self.f_rstd = (np.ascontiguousarray(x, np.float32) for x in (params, f_mean, f_rstd))
+        self.lo, self.hi, self.o_mean, self.o_std = (np.ascontiguousarray(x, np.float32) for x in (lo, hi, o_mean, o_std))
+
+    def save(self, path):
+        np.savez(path, width=self.width, depth=self.depth, **{k: getattr(self, k) for k in self.FIELDS})
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        return cls(int(z["width"]), int(z["depth"]), *(z[k] for k in cls.FIELDS))
+
+
+def features(T, P, Y):
+    """[n][K+2] float64: (T, ln P, Y_0..Y_{K-1})."""
+    return np.concatenate([np.asarray(T)[:, None], np.log(np.asarray(P))[:, None], np.asarray(Y).T], axis=1)
+
+
+def log_baseline(mechanism, T, P, D=4, T0=1000.0):
+    """ln b_o [n][K+2] (see the module docstring)."""
+    c, v, d = mechanism.transport_fits(D, T0)
+    K = mechanism.K
+    b = K - 1
+    T = np.asarray(T, np.float64)
+    l = np.log(T / T0)
+    pw = np.stack([l ** i for i in range(D)], axis=1)
+    out = np.empty((len(T), K + 2))
+    out[:, 0] = np.log(pw @ c[b]) + 0.5 * np.log(T)
+    out[:, 1] = 2 * np.log(pw @ v[b]) + 0.5 * np.log(T)
+    partner = np.full(K, b)
+    partner[b] = 0
+    dk = d[np.arange(K), partner]  # [K][D]
+    out[:, 2:] = np.log(pw @ dk.T) + 1.5 * np.log(T)[:, None] - np.log(np.asarray(P))[:, None]
+    return out
+
+
+def judge(mechanism, T, P, Y, device="cuda", chunk=1 << 20):
+    """The plain FP64 transport outputs [K+2][n] (tiled FP64 form on the GPU)."""
+    import torch
+    prog = mechanism.chemistry(native.TRANSPORT)
+    mod = prog.assemble(form="fp64")
+    outs = []
+    for i in range(0, len(T), chunk):
+        ins = [torch.from_numpy(np.ascontiguousarray(x[i:i + chunk])).to(device) for x in (T, P)]
+        ins += [torch.from_numpy(np.ascontiguousarray(y[i:i + chunk])).to(device) for y in Y]
+        outs.append(torch.stack(mod(ins)).cpu().numpy())
+    return np.concatenate(outs, axis=1)
+
+
+def initial_params(net, rng):
+    """LeCun-normal weights on the valid inputs, zero biases and zero padding."""
+    params = np.zeros(net.n_params, np.float32)
+    for l, (w, b, o, i) in enumerate(net.layers):
+        fan_in = net.n_in if l == 0 else i
+        W = rng.normal(0, 1 / math.sqrt(fan_in), (o, i)).astype(np.float32)
+        if l == 0:
+            W[:, net.n_in:] = 0
+        if l == len(net.layers) - 1:
+            W[net.n_out:] = 0
+        params[w:w + o * i] = W.ravel()
+    return params
+
+
+def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384, lr=2e-3, seed=0, device="cuda", log=None):
+    """Fits the network to props [K+2][n] (the judge's outputs) on the GPU with the project's kernels."""
+    import torch
+    K = mechanism.K
+    NF = K + 2
+    F = features(T, P, Y)
+    R = np.log(np.asarray(props).T) - log_baseline(mechanism, T, P)
+    f_mean, f_std = F.mean(0), F.std(0)
+    f_rstd = np.where(f_std > 0, 1 / np.maximum(f_std, 1e-300), 0.0)
+    o_mean, o_std = R.mean(0), np.maximum(R.std(0), 1e-6)
+    net = native.Network(NF, width, depth, NF)
+    X = np.zeros((len(T), net.in_pad), np.float32)
+    X[:, :NF] = (F - f_mean) * f_rstd
+    Tg = np.zeros((len(T), net.out_pad), np.float32)
+    Tg[:, :NF] = (R - o_mean) / o_std
+    dev = torch.device(device)
+    Xd, Td = torch.from_numpy(X).to(dev), torch.from_numpy(Tg).to(dev)
+    rng = np.random.default_rng(seed)
+    p = torch.from_numpy(initial_params(net, rng)).to(dev)
+    p16 = p.to(torch.bfloat16)
+    m1, m2, grad = torch.zeros_like(p), torch.zeros_like(p), torch.zeros_like(p)
+    y = torch.empty((batch, net.out_pad), dtype=torch.float32, device=dev)
+    ws = torch.empty(net.workspace_bytes(batch), dtype=torch.uint8, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    n = len(T)
+    perm = torch.randperm(n, generator=gen, device=dev)
+    at = 0
+    history = []
+    for step in range(1, steps + 1):
+        if at + batch > n:
+            perm = torch.randperm(n, generator=gen, device=dev)
+            at = 0
+        idx = perm[at:at + batch]
+        at += batch
+        xb, tb = Xd[idx].contiguous(), Td[idx].contiguous()
+        net.forward(p, p16, xb, y, ws)
+        net.backward_mse(p16, y, tb, ws, grad)
+        rate = lr * 0.5 * (1 + math.cos(math.pi * step / steps)) + 1e-5
+        net.adam(p, p16, grad, m1, m2, rate, step)
+        if step % 500 == 0 or step == steps:
+            loss = float(((y[:, :NF] - tb[:, :NF]) ** 2).mean())
+            history.append((step, loss))
+            if log:
+                log(f"step {step}: batch loss {loss:.3e}")
+    trained = Trained(width, depth, p.cpu().numpy(), f_mean, f_rstd, F.min(0), F.max(0), o_mean, o_std)
+    trained.history = history
+    return trained
+
+
+def assemble(program, trained):
+    """The surrogate form of a chemistry program (cb_hip_assemble_surrogate) as a native.Module."""
+    K = program.mechanism.K
+    arrays = [np.ascontiguousarray(getattr(trained, k), np.float32) for k in Trained.FIELDS]
+    for a in arrays[1:]:
+        if a.shape != (K + 2,):
+            raise ValueError("normalisation arrays need K+2 entries")
+    fp = C.POINTER(C.c_float)
+    w = native.SurrogateWeights(trained.width, trained.depth, *(a.ctypes.data_as(fp) for a in arrays))
+    h = native._P()
+    native._check(native.lib().cb_hip_assemble_surrogate(program._h, C.byref(w), C.byref(h)))
+    mod = native.Module(h, program)
+    mod._keep = arrays
+    return mod
+
+
+def fallback_count(module, stream=None):
+    c = C.c_uint32()
+    native._check(native.lib().cb_module_fallback_count(module._h, stream, C.byref(c)))
+    return c.value
+
+
+def main(argv=None):
+    """Generates trajectory states, computes the judge's transport on them, trains the network and saves it:
+    python -m combustion_amd.surrogate --mechanism gri-53 --out surrogate_gri53.npz"""
+    import argparse
+    import time
+
+    from . import reactor, synth
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mechanism", default="gri-53")
+    ap.add_argument("--trajectories", type=int, default=8192)
+    ap.add_argument("--states", type=int, default=1 << 22)
+    ap.add_argument("--max-steps", type=int, default=1200)
+    ap.add_argument("--t-end", type=float, default=1e-3)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--width", type=int, default=256)
+    ap.add_argument("--depth", type=int, default=3)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    log = lambda m: print(f"[{time.strftime('%H:%M:%S')}] {m}", flush=True)
+    mech = native.Mechanism(synth.mechanism_path(a.mechanism))
+    t = time.time()
+    T, P, Y = reactor.trajectories(mech, a.trajectories, a.states, a.seed, t_end=a.t_end, max_steps=a.max_steps, log=log)
+    log(f"{len(T)} training states from {a.trajectories} trajectories in {time.time() - t:.1f} s; T in [{T.min():.0f}, {T.max():.0f}]")
+    t = time.time()
+    props = judge(mech, T, P, Y)
+    log(f"judge transport in {time.time() - t:.1f} s")
+    t = time.time()
+    trained = train(mech, T, P, Y, props, width=a.width, depth=a.depth, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    log(f"trained in {time.time() - t:.1f} s")
+    trained.save(a.out)
+    log(f"saved {a.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -100,6 +100,49 @@ int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, s
                 int32_t pointers_are_device, void* stream, float* kernel_ms);
 void cb_module_free(cb_module* mod);
 
+/* Transport surrogate network (no counterpart in the reference, whose transport is always evaluated by the
+ * transport.rs formulas; this is the learned replacement of that section). An MLP n_in -> width (SiLU) x depth ->
+ * n_out with BF16 weights and FP32 accumulation on MFMA. Parameters are one flat FP32 array (FP32 master copy)
+ * plus its BF16 copy, both device memory: per layer W [out][in] then b [out], with in/out padded to multiples
+ * of 64 (cb_mlp_sizes). Activations are [n][in_pad] / [n][out_pad] FP32 row-major device arrays. */
+/* Surrogate transport form of a chemistry program (inputs [T, P, Y], CB_TRANSPORT with or without CB_RATES):
+ * ω by the mixed-precision rates kernel, and λ, μ, D_k by the trained network (kernels/surrogate.hip), with the
+ * FP32 transport kernel evaluating the states whose features fall outside the training box [lower, upper].
+ * No counterpart in the reference. The network has n_in = n_out = K+2 (cb_mlp layout); features are
+ * (T, ln P, Y_0..Y_{K-1}) standardised by (mean, 1/std); outputs are [λ, μ, D_0..D_{K-1}] as
+ * ln(property / baseline) standardised by (mean, std), the baseline being the bulk species' conductivity and
+ * viscosity fits and the binary diffusion fit against the bulk species (species 0 for the bulk itself). */
+typedef struct cb_surrogate_weights {
+  int32_t width, depth;
+  const float* params;          /* cb_mlp_sizes(K+2, width, depth, K+2) n_params values */
+  const float* feature_mean;    /* [K+2] */
+  const float* feature_rstd;    /* [K+2] */
+  const float* lower;           /* [K+2] training box, in feature units (T, ln P, Y) */
+  const float* upper;           /* [K+2] */
+  const float* output_mean;     /* [K+2] */
+  const float* output_std;      /* [K+2] */
+} cb_surrogate_weights;
+int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w, cb_module** out);
+/* States of the last cb_hip_call on a surrogate module that went to the plain transport kernel (waits for the
+ * module's last stream). */
+int cb_module_fallback_count(cb_module* mod, void* stream, uint32_t* count);
+
+typedef struct cb_mlp cb_mlp;
+int cb_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, cb_mlp** out);
+void cb_mlp_free(cb_mlp* mlp);
+/* n_params: flat parameter count; in_pad/out_pad: padded widths; layer_offsets[2 (depth+1)]: W and b offsets. */
+int cb_mlp_sizes(const cb_mlp* mlp, int64_t* n_params, int32_t* in_pad, int32_t* out_pad, int64_t* layer_offsets);
+int cb_mlp_workspace_bytes(const cb_mlp* mlp, int64_t n, size_t* bytes);
+/* Forward pass (keeps activations in the workspace for cb_mlp_backward_mse). */
+int cb_mlp_forward(const cb_mlp* mlp, const float* params, const uint16_t* params_bf16, const float* x, int64_t n, float* y,
+                   void* workspace, void* stream);
+/* Gradient of the mean squared error over the n x n_out valid outputs, written to grad (flat FP32). */
+int cb_mlp_backward_mse(const cb_mlp* mlp, const uint16_t* params_bf16, const float* y, const float* target, int64_t n,
+                        void* workspace, float* grad, void* stream);
+/* One Adam step (step counts from 1) on the FP32 parameters; refreshes the BF16 copy. */
+int cb_mlp_adam(const cb_mlp* mlp, float* params, uint16_t* params_bf16, const float* grad, float* m1, float* m2, float lr,
+                float beta1, float beta2, float eps, int32_t step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

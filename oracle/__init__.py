@@ -8,9 +8,9 @@ baseline. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg m
   chem.c        closed-form FP64 rates, NASA-7 and mixture-averaged transport (reaction.rs, transport.rs)
   interpret.c   FP64 interpreter of a program's linear form (ast/interpret.rs:52-148)
   fits.py       kinetic-theory species properties and polynomial fits (transport.rs:14-137)
+  mlp.py        FP64 forward/backward of the surrogate network with the kernels' BF16 rounding points
 
-Pinning status of each function is listed in DESIGN.md's oracle section; the tests that pin them are in
-tests/test_oracle_*.py.
+The tests that pin each function are in tests/test_oracle_*.py.
 """
 import ctypes as C
 import os
