@@ -313,6 +313,43 @@ int cb_program_linear(const cb_program* p, uint8_t* op, int32_t* dst, int32_t* a
   });
 }
 
+int cb_program_from_linear(int32_t n_inputs, int64_t n_instructions, const uint8_t* op, const int32_t* dst, const int32_t* a,
+                           const int32_t* b, const int32_t* c, const double* k, int32_t n_outputs, const int32_t* outputs,
+                           cb_program** out) {
+  return guard([&] {
+    require(out && n_inputs >= 0 && n_instructions >= 0 && n_outputs >= 0, "bad argument");
+    require(n_instructions == 0 || (op && dst && a && b && c && k), "null instruction array");
+    require(n_outputs == 0 || outputs, "null outputs");
+    ast::Linear l;
+    l.n_input = n_inputs;
+    int32_t next = n_inputs;  // SSA: each instruction defines the next register
+    auto defined = [&](int32_t r) { return r >= 0 && r < next; };
+    for (int64_t i = 0; i < n_instructions; i++) {
+      require(op[i] <= (uint8_t)ast::LOp::Select, "unknown op code");
+      const auto o = (ast::LOp)op[i];
+      require(dst[i] == next, "instructions must define registers n_inputs, n_inputs+1, ... in order");
+      const int arity = o == ast::LOp::Const ? 0 : o == ast::LOp::Select ? 3
+                        : (o == ast::LOp::Neg || o == ast::LOp::Sqrt || o == ast::LOp::Exp || o == ast::LOp::Ln || o == ast::LOp::Sq) ? 1 : 2;
+      require(arity < 1 || defined(a[i]), "operand a is not a defined register");
+      require(arity < 2 || defined(b[i]), "operand b is not a defined register");
+      require(arity < 3 || defined(c[i]), "operand c is not a defined register");
+      l.code.push_back({o, dst[i], a[i], b[i], c[i], k[i]});
+      next++;
+    }
+    l.n_registers = next;
+    for (int32_t i = 0; i < n_outputs; i++) {
+      require(defined(outputs[i]), "output is not a defined register");
+      l.output.push_back(outputs[i]);
+    }
+    auto* p = new cb_program;
+    p->function.n_input = n_inputs;
+    p->function.output.assign(n_outputs, -1);  // the program lives in its linear form only
+    p->linear = std::move(l);
+    p->kind = ProgramKind::Other;
+    *out = p;
+  });
+}
+
 int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed) {
   std::string text;
   int status = guard([&] {

@@ -12,7 +12,7 @@
 //   4. ω += ν·q as split-BF16 MFMA products (FP32 accumulation) under a per-state block-floating-point exponent,
 //      since |q| exceeds the FP32 range on some states.
 // Required defines: K, KA, KB (species k-blocks of 16, over K), AB (active-species tiles of 16), RT (reaction tiles),
-// KE (species blocks of 4 for the third-body contraction).
+// KE (species blocks of 4 for the third-body contraction), WAVES (waves per block, sized to the LDS).
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -22,7 +22,6 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 #define NS 16
-#define WAVES 4
 #define LOG2E 1.4426950408889634
 #define LN2F 0.6931471805599453f
 #define LN10 2.302585092994046

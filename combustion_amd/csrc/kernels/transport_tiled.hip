@@ -15,7 +15,10 @@
 // F64 (1: FP64 arithmetic and v_mfma_f64_16x16x4f64; 0: FP32 arithmetic and v_mfma_f32_16x16x4f32).
 // With a state list (idx, count) the kernel evaluates only the listed states, in place.
 
+// Output row of (lane group g, element i) in the 16x16 MFMA result: FP32 holds rows 4g..4g+3 in a lane,
+// the FP64 instruction interleaves them (rows g, g+4, g+8, g+12).
 #if F64
+#define ROW(g, i) ((g) + 4 * (i))
 typedef double real;
 typedef double r4 __attribute__((ext_vector_type(4)));
 #define MFMA(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0)
@@ -28,6 +31,7 @@ __device__ __forceinline__ real rcp(real b) {
 }
 __device__ __forceinline__ real sqrt_real(double x) { return __builtin_sqrt(x); }
 #else
+#define ROW(g, i) (4 * (g) + (i))
 typedef float real;
 typedef float r4 __attribute__((ext_vector_type(4)));
 #define MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0)
@@ -134,7 +138,7 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
   B = block_sum(L, 3, B, wave, grp, col);
   WN = block_sum(L, 0, WN, wave, grp, col);
 
-  // 2. Viscosity: rows k = 16 mt + 4 grp + i of the MFMA output, state = col.
+  // 2. Viscosity: rows k = 16 mt + ROW(grp, i) of the MFMA output, state = col.
   real mu = 0;
   for (int mt = wave; mt < KP / 16; mt += WAVES) {
     r4 d1 = {0, 0, 0, 0}, d2 = d1, d3 = d1;
@@ -149,7 +153,7 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int k = 16 * mt + 4 * grp + i;
+      const int k = 16 * mt + ROW(grp, i);
       if (k < K) {
         const real v = rcp(L.r[k][col]);
         const real den = d1[i] + 2 * v * d2[i] + v * v * d3[i];
@@ -158,7 +162,7 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
     }
   }
 
-  // 3. Binary diffusion: tile t = (k, j0) gives rows j = j0 + 4 grp + i.
+  // 3. Binary diffusion: tile t = (k, j0) gives rows j = j0 + ROW(grp, i).
   const real lp0 = L.pw[grp][col];
   const real lp1 = DK > 4 ? L.pw[(4 + grp) % DK][col] : (real)0;
   real (*S)[NS] = L.S[wave];
@@ -171,7 +175,7 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
     real sk = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int j = j0 + 4 * grp + i;
+      const int j = j0 + ROW(grp, i);
       if (j < k) {
         const real R = rcp(d[i]);
         sk += L.x[j][col] * R;

@@ -28,7 +28,7 @@ uint16_t bf16_bits(float x) {
 
 // Tables of kernels/rates_mixed.hip (layouts described there).
 struct MixedTables {
-  int K, KA, KB, AB, RT, KE;
+  int K, KA, KB, AB, RT, KE, WAVES;
   std::vector<uint16_t> A_fwd, A_rev, A_nu;
   std::vector<double> rate, k0;
   std::vector<float> troe, eff;
@@ -166,6 +166,10 @@ MixedTables mixed_tables(const Mechanism& m) {
     }
     t.split.push_back(std::isnan(n.temperature_split) ? INFINITY : n.temperature_split);
   }
+  // LDS per wave (struct Lds of rates_mixed.hip): as many waves per block (up to 4) as fit in 160 KiB.
+  const size_t per_wave = (size_t)t.KB * 3 * 64 * 4 * 2 + (size_t)(t.KA + 1) * 16 * 8 + (size_t)t.KE * 4 * 16 * 4 + 16 * (3 * 8 + 2 * 4);
+  t.WAVES = (int)std::min<size_t>(4, 163840 / per_wave);
+  if (t.WAVES < 1) throw Unsupported("mechanism too large for the mixed rates kernel's LDS");
   t.ln_clamp_pos = std::log(6981463048192.);  // reaction.rs:244-256
   t.ln_clamp_neg = std::log((double)cbrtf(3.40282347e38f));
   return t;
@@ -182,7 +186,7 @@ T* upload(const std::vector<T>& v) {
 std::string rates_mixed_text(const MixedTables& t) {
   std::ostringstream s;
   s << "#define K " << t.K << "\n#define KA " << t.KA << "\n#define KB " << t.KB << "\n#define AB " << t.AB << "\n#define RT " << t.RT
-    << "\n#define KE " << t.KE << "\n"
+    << "\n#define KE " << t.KE << "\n#define WAVES " << t.WAVES << "\n"
     << rates_mixed_source;
   return s.str();
 }
@@ -342,12 +346,12 @@ class RatesMixed {
   }
   void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
     unsigned n32 = (unsigned)n;
-    const unsigned states_per_block = 16 * 4;
+    const unsigned states_per_block = 16 * t_.WAVES;
     unsigned grid = (unsigned)((n + states_per_block - 1) / states_per_block);
     double pos = t_.ln_clamp_pos, neg = t_.ln_clamp_neg;
     void* args[] = {&in, &out, &n32, &d_fwd_, &d_rev_, &d_nu_, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_tile_, &d_eff_,
                     &d_rcp_W_, &d_gibbs_, &d_split_, &pos, &neg};
-    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch rates_mixed");
+    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 64 * t_.WAVES, 1, 1, 0, stream, args, nullptr), "launch rates_mixed");
   }
 
  private:

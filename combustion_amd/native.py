@@ -60,6 +60,8 @@ SIGNATURES = {
     "cb_program_sizes": (C.c_int, [_P, _I32P, _I32P, _I64P, _I64P, _I32P]),
     "cb_program_op_counts": (C.c_int, [_P, _I64P]),
     "cb_program_linear": (C.c_int, [_P, C.POINTER(C.c_uint8), _I32P, _I32P, _I32P, _I32P, _DP, _I32P]),
+    "cb_program_from_linear": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_uint8), _I32P, _I32P, _I32P, _I32P, _DP, C.c_int32, _I32P,
+                                         C.POINTER(_P)]),
     "cb_program_hip_source": (C.c_int, [_P, C.c_int32, C.c_uint32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "cb_hip_assemble": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
@@ -219,6 +221,19 @@ class Program:
                                        k.ctypes.data_as(_DP), out.ctypes.data_as(_I32P)))
         return {"op": op, "dst": dst, "a": a, "b": b, "c": c, "k": k, "outputs": out, "n_inputs": self.n_inputs,
                 "n_registers": self.n_registers}
+
+    @classmethod
+    def from_linear(cls, linear):
+        """cb_program_from_linear: a program from a linear form (the dict that linear() returns)."""
+        op = np.ascontiguousarray(linear["op"], np.uint8)
+        dst, a, b, c, out = (np.ascontiguousarray(linear[x], np.int32) for x in ("dst", "a", "b", "c", "outputs"))
+        k = np.ascontiguousarray(linear["k"], np.float64)
+        h = _P()
+        _check(lib().cb_program_from_linear(int(linear["n_inputs"]), len(op), op.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                            dst.ctypes.data_as(_I32P), a.ctypes.data_as(_I32P), b.ctypes.data_as(_I32P),
+                                            c.ctypes.data_as(_I32P), k.ctypes.data_as(_DP), len(out), out.ctypes.data_as(_I32P),
+                                            C.byref(h)))
+        return cls(h, None)
 
     def hip_source(self, form="straight", block_size=0):
         return _text(lib().cb_program_hip_source, self._h, FORMS[form], block_size)

@@ -128,7 +128,7 @@ def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384
         net.backward_mse(p16, y, tb, ws, grad)
         rate = lr * 0.5 * (1 + math.cos(math.pi * step / steps)) + 1e-5
         net.adam(p, p16, grad, m1, m2, rate, step)
-        if step % 500 == 0 or step == steps:
+        if step == 1 or step % 500 == 0 or step == steps:
             loss = float(((y[:, :NF] - tb[:, :NF]) ** 2).mean())
             history.append((step, loss))
             if log:

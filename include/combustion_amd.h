@@ -85,6 +85,12 @@ int cb_program_op_counts(const cb_program* p, int64_t counts[8]);
  * 0 const,1 neg,2 min,3 max,4 add,5 sub,6 le,7 mul,8 div,9 sqrt,10 exp,11 ln,12 sq,13 select(a ? b : c).
  * Arrays have `instructions` entries; outputs has `outputs` entries (registers). */
 int cb_program_linear(const cb_program* p, uint8_t* op, int32_t* dst, int32_t* a, int32_t* b, int32_t* c, double* k, int32_t* outputs);
+/* The inverse of cb_program_linear: a program given in the linear form (same op codes and operand meaning),
+ * for callers that build their own AST (the reference's ast::Function, linearised by its caller) and want the
+ * straight-line HIP forms. Instruction i must define register n_inputs + i; operands refer to earlier registers. */
+int cb_program_from_linear(int32_t n_inputs, int64_t n_instructions, const uint8_t* op, const int32_t* dst, const int32_t* a,
+                           const int32_t* b, const int32_t* c, const double* k, int32_t n_outputs, const int32_t* outputs,
+                           cb_program** out);
 /* HIP text of a form (cuda.rs:101 compile counterpart). */
 int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed);
 

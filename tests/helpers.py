@@ -30,5 +30,6 @@ def state_floor(scale, Ka, ratio=1e-30):
     be represented; the stated floor puts them below 1e-30 of the state's largest term scale."""
     import numpy as np
     f = scale.copy()
-    f[:Ka] = np.maximum(scale[:Ka], ratio * scale[:Ka].max(axis=0, keepdims=True))
+    if Ka:
+        f[:Ka] = np.maximum(scale[:Ka], ratio * scale[:Ka].max(axis=0, keepdims=True))
     return f

@@ -1,5 +1,11 @@
 """Surrogate network kernels (kernels/mlp.hip) against the CPU oracle (oracle/mlp.py, which reproduces the
-kernels' BF16 rounding points): forward outputs and the gradients of the mean squared error."""
+kernels' BF16 rounding points): forward outputs and the gradients of the mean squared error.
+
+The oracle sums in FP64 and the kernels in FP32, so an activation that lies within an FP32 rounding of a BF16
+rounding boundary rounds the other way in one of them (about 1 in 10^5 activations of the first hidden layer,
+growing through the layers); such a flip moves that activation by one BF16 unit (2^-8 relative). The bars are
+therefore on the 99th percentile (1e-3 of the largest value: a systematic error such as a wrong operand layout
+moves every element by O(1)) and on the maximum (2e-2, a handful of flips)."""
 import numpy as np
 import pytest
 
@@ -43,17 +49,17 @@ def run_network(n, seed, gpu):
 def test_forward_and_gradients_match_oracle(n, gpu):
     net, params, x, t, y, grad = run_network(n, 3, gpu)
     ref_grad, ref_y = omlp.backward_mse(params, net.layers, x, t, 55)  # rounds W to BF16 as the kernels' copy does
-    ey = np.abs(y[:, :55] - ref_y[:, :55]).max() / np.abs(ref_y[:, :55]).max()
-    print(f"n={n}: output max error {ey:.2e}", flush=True)
-    assert ey < 2e-3
+    def check(name, got, ref):
+        e = np.abs(got - ref) / np.abs(ref).max()
+        p99, mx = np.percentile(e, 99), e.max()
+        print(f"n={n} {name}: 99th percentile {p99:.2e}, max {mx:.2e}", flush=True)
+        assert p99 < 1e-3 and mx < 2e-2
+
+    check("output", y[:, :55], ref_y[:, :55])
     assert np.all(y[:, 55:] == 0)
     for l, (w, b, o, i) in enumerate(net.layers):
-        gw, rw = grad[w:w + o * i], ref_grad[w:w + o * i]
-        gb, rb = grad[b:b + o], ref_grad[b:b + o]
-        ew = np.abs(gw - rw).max() / np.abs(rw).max()
-        eb = np.abs(gb - rb).max() / np.abs(rb).max()
-        print(f"layer {l}: dW max error {ew:.2e}, db max error {eb:.2e}", flush=True)
-        assert ew < 5e-3 and eb < 5e-3
+        check(f"layer {l} dW", grad[w:w + o * i], ref_grad[w:w + o * i])
+        check(f"layer {l} db", grad[b:b + o], ref_grad[b:b + o])
 
 
 @pytest.mark.gpu

@@ -100,3 +100,29 @@ def test_interpreter_reports_nan_like_the_reference():
     T[2] = np.nan
     with pytest.raises(FloatingPointError, match="state 2"):
         oracle.interpret(M.chemistry().linear(), np.vstack([T, P, Y]))
+
+
+def test_program_round_trips_through_the_linear_form():
+    """cb_program_from_linear(cb_program_linear(p)) emits the same straight-line HIP and interprets the same."""
+    import oracle
+    from combustion_amd.native import Program
+    M, mech, o = setup("h2o2-10")
+    p = M.chemistry()
+    lin = p.linear()
+    q = Program.from_linear(lin)
+    assert (q.n_inputs, q.n_outputs, q.n_instructions) == (p.n_inputs, p.n_outputs, p.n_instructions)
+    assert q.hip_source("straight") == p.hip_source("straight")
+    T, P, Y = synth.config_states("h2o2-10", 3, 1)
+    ins = np.vstack([T, P, Y])
+    assert np.array_equal(oracle.interpret(q.linear(), ins), oracle.interpret(lin, ins))
+
+
+def test_program_from_linear_rejects_bad_programs():
+    import pytest
+    from combustion_amd.native import Error, Program
+    bad = {"op": [4], "dst": [2], "a": [0], "b": [5], "c": [0], "k": [0.0], "outputs": [2], "n_inputs": 2}
+    with pytest.raises(Error, match="operand b"):
+        Program.from_linear(bad)
+    bad = {"op": [0], "dst": [3], "a": [0], "b": [0], "c": [0], "k": [1.0], "outputs": [2], "n_inputs": 2}
+    with pytest.raises(Error, match="in order"):
+        Program.from_linear(bad)
