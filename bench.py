@@ -1,10 +1,14 @@
 """Throughput of per-state rates + mixture-averaged transport on one MI355X.
 
-Workload: the 53-species / 325-reaction synthetic mechanism (30 three-body, 25 Troe), 2^20 states in SoA
-layout (T ~ U[600, 2800] K, P ~ U[1, 40] atm, Y ~ Dirichlet(0.3)), outputs ω_k, λ, μ, D_k. A step is one pass
-of the chemistry kernel(s) over all states, inputs already in HBM. Prints one JSON line.
+Workload: the 53-species / 325-reaction synthetic mechanism (30 three-body, 25 Troe), 2^20 states in SoA layout
+(T ~ U[600, 2800] K, P ~ U[1, 40] atm, Y ~ Dirichlet(0.3)), outputs ω_k, λ, μ, D_k. A step is one pass of a
+form's kernels over all states, inputs already in HBM. `value` is the plain FP64 form; the line also carries the
+mixed-precision form and the mixed + surrogate form (when a trained network is present) with their speedups and
+their errors against the FP64 oracle on a sample of the same states. Prints one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--form fp64|straight|mixed] [--states N] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--states N] [--no-cpu] [--surrogate PATH]
+  python bench.py --only rates|transport|fp64|mixed|copy [--steps K]   (one launch kind alone, for rocprofv3 runs;
+      copy is a device-to-device copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
 """
 import argparse
 import json
@@ -20,6 +24,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 MECHANISM = "gri-53"
+SURROGATE = os.path.join(ROOT, "combustion_amd", "data", "surrogate_gri53.npz")
 
 
 def log(*a):
@@ -43,8 +48,8 @@ def cpu_baseline(name, budget_s=12.0):
     o.batch(T, P, Y, threads=threads)
     rate = n / (time.perf_counter() - t)
     return {"value": rate, "unit": "states/s", "cores": threads, "kind": "port",
-            "sample": f"{n} states of the same workload, FP64 oracle (oracle/chem.c) on {threads} host threads; "
-                      f"single-thread {single:.0f} states/s"}
+            "sample": f"{n} states of the same workload (rates + transport, K=53/R=325), FP64 oracle (oracle/chem.c) on "
+                      f"{threads} host threads; single-thread {single:.0f} states/s"}
 
 
 def traffic_from_profiles(kernel):
@@ -55,15 +60,48 @@ def traffic_from_profiles(kernel):
     return None
 
 
+def time_module(mod, inputs, outputs, steps, warmup):
+    """(wall seconds for `steps` calls, mean kernel ms by HIP events on the launch stream)."""
+    import torch
+    for _ in range(warmup):
+        mod(inputs, outputs)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mod(inputs, outputs, time_kernel=True)
+        kernel_ms.append(mod.last_kernel_ms)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, float(np.mean(kernel_ms))
+
+
+def errors_vs_oracle(out_dev, T, P, Y, n_sample, Ka, rates=True):
+    """Per-section max and median relative error of a form's outputs on the first n_sample states."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import relative_error, setup, state_floor
+    _, _, o = setup(MECHANISM)
+    ref, floor = o.batch(T[:n_sample], P[:n_sample], Y[:, :n_sample], rates=rates, threads=16)
+    got = np.stack([x[:n_sample].cpu().numpy() for x in out_dev])
+    f = state_floor(floor, Ka if rates else 0)
+    err = relative_error(got, ref, f)
+    k = Ka if rates else 0
+    sec = {"λ": err[k], "μ": err[k + 1], "D": err[k + 2:]}
+    if rates:
+        sec = {"ω": err[:Ka], **sec}
+    return {name: {"max": float(e.max()), "median": float(np.median(e))} for name, e in sec.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", default="auto")
     ap.add_argument("--states", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--surrogate", default=SURROGATE)
+    ap.add_argument("--error-sample", type=int, default=8192)
+    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "copy"])
+    ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every state is independent, so N GPUs would each run "
@@ -72,61 +110,99 @@ def main():
 
     import torch
     from combustion_amd import Mechanism, synth
+    from combustion_amd import surrogate as sur
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
     M = Mechanism(synth.mechanism_path(MECHANISM))
-    prog = M.chemistry()
-    counts = prog.op_counts()
-    form = args.form
-    if form == "auto":
-        form = "fp64"
-        try:
-            mod = prog.assemble(args.block_size, form=form)
-        except Exception as e:
-            log("fp64 form unavailable:", e)
-            form = "straight"
-            mod = prog.assemble(args.block_size, form=form)
-    else:
-        mod = prog.assemble(args.block_size, form=form)
+    Ka = M.active
     N = args.states
     T, P, Y = synth.config_states(MECHANISM, N, 2024)
     inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
+    prog = M.chemistry()
     outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        mod(inputs, outputs)
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mod(inputs, outputs, time_kernel=True)  # HIP events on the launch stream around the kernel(s)
-        kernel_ms.append(mod.last_kernel_ms)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    assert all(torch.isfinite(o).all().item() for o in outputs[:4])
-    ms_per_step = elapsed / args.steps * 1e3
-    value = N * args.steps / elapsed
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    flop_per_state = counts["weighted"]
-    achieved = flop_per_state * N / avg_kernel_s / 1e12
+    if args.only:
+        return only(args, M, inputs, N)
+
+    # The plain FP64 form: the whole step, then each of its two kernels alone (rates; transport) for the roofline.
+    fp64 = prog.assemble(form="fp64")
+    wall, step_ms = time_module(fp64, inputs, outputs, args.steps, args.warmup)
+    assert all(torch.isfinite(o).all().item() for o in outputs)
+    value = N * args.steps / wall
+    sections = {}
+    for name, flags in (("rates", 1), ("transport", 2)):
+        p = M.chemistry(flags)
+        mod = p.assemble(form="fp64")
+        outs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(p.n_outputs)]
+        _, ms = time_module(mod, inputs, outs, max(5, args.steps // 2), 2)
+        flop = p.op_counts()["weighted"]
+        sections[name] = {"kernel": "rates_fp64" if flags == 1 else "transport_tiled", "kernel_ms": ms,
+                          "flop_per_state": flop, "achieved_tflops": flop * N / (ms * 1e-3) / 1e12}
+        del outs
+    dominant = max(sections.values(), key=lambda s: s["kernel_ms"])
+    flop_total = prog.op_counts()["weighted"]
     hbm_bytes = 8 * N * (prog.n_inputs + prog.n_outputs)
+    achieved = dominant["achieved_tflops"]
     result = {
         "metric": "states/s rates+transport, K=53/R=325: FP64 plain vs mixed+surrogate at stated err",
         "value": value, "unit": "states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded mechanism and states)",
         "config": {"workload": "rates+transport, K=53 species / R=325 reactions (30 three-body, 25 Troe), 2^20 states",
-                   "states": N, "form": form, "mechanism_seed": synth.CONFIGS[MECHANISM][4]},
+                   "states": N, "form": "fp64", "mechanism_seed": synth.CONFIGS[MECHANISM][4]},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(f"chemistry_{form}"),
-                     "flop_per_state": flop_per_state, "kernel_ms": avg_kernel_s * 1e3,
-                     "hbm_algorithmic_bytes": hbm_bytes, "hbm_frac": hbm_bytes / avg_kernel_s / 1e9 / PEAK_HBM_GBS},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(dominant["kernel"]),
+                     "kernel": dominant["kernel"], "flop_per_state": dominant["flop_per_state"],
+                     "kernel_ms": dominant["kernel_ms"],
+                     "step": {"kernel_ms": step_ms, "flop_per_state": flop_total,
+                              "achieved": flop_total * N / (step_ms * 1e-3) / 1e12,
+                              "frac": flop_total * N / (step_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                              "hbm_algorithmic_bytes": hbm_bytes, "hbm_frac": hbm_bytes / (step_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+                     "sections": sections},
     }
+    forms = {"fp64": {"states_per_s": value, "kernel_ms": step_ms}}
+
+    mixed = prog.assemble(form="mixed")
+    wall_m, ms_m = time_module(mixed, inputs, outputs, args.steps, args.warmup)
+    forms["mixed"] = {"states_per_s": N * args.steps / wall_m, "kernel_ms": ms_m, "speedup_vs_fp64": wall / wall_m,
+                      "error_vs_oracle": errors_vs_oracle(outputs, T, P, Y, args.error_sample, Ka)}
+    if os.path.exists(args.surrogate):
+        trained = sur.Trained.load(args.surrogate)
+        smod = sur.assemble(prog, trained)
+        wall_s, ms_s = time_module(smod, inputs, outputs, args.steps, args.warmup)
+        forms["mixed_surrogate"] = {
+            "states_per_s": N * args.steps / wall_s, "kernel_ms": ms_s, "speedup_vs_fp64": wall / wall_s,
+            "fallback_states": sur.fallback_count(smod),
+            "note": "the benchmark states are drawn over the whole T/P/Y box, not from reactor trajectories, so most "
+                    "of them fall outside the network's training box and take the plain FP32 transport kernel",
+            "error_vs_oracle": errors_vs_oracle(outputs, T, P, Y, args.error_sample, Ka)}
+    result["forms"] = forms
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(MECHANISM)
     print(json.dumps(result))
+
+
+def only(args, M, inputs, N):
+    """One launch kind alone, args.steps times (the command that rocprofv3 summaries in profiles/ are made of)."""
+    import torch
+    if args.only == "copy":
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=inputs[0].device)
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"only": "copy", "bytes_read_per_launch": src.numel() * 4, "bytes_written_per_launch": dst.numel() * 4,
+                          "steps": args.steps}))
+        return
+    flags = {"rates": 1, "transport": 2, "fp64": 3, "mixed": 3}[args.only]
+    p = M.chemistry(flags)
+    mod = p.assemble(form="mixed" if args.only == "mixed" else "fp64")
+    outs = [torch.empty(N, dtype=torch.float64, device=inputs[0].device) for _ in range(p.n_outputs)]
+    wall, ms = time_module(mod, inputs, outs, args.steps, 1)
+    print(json.dumps({"only": args.only, "states": N, "steps": args.steps, "kernel_ms": ms,
+                      "flop_per_state": p.op_counts()["weighted"],
+                      "algorithmic_hbm_bytes_per_launch": 8 * N * (p.n_inputs + p.n_outputs)}))
 
 
 if __name__ == "__main__":

@@ -10,6 +10,10 @@ struct StraightOptions {
   unsigned block_size = 256;
   bool reciprocal = false;   // a / b as a * (1/b), 1/b by v_rcp_f64 and two Newton steps (within 1 ulp of 1/b)
   bool f32 = false;          // evaluate in FP32 (inputs and outputs stay FP64 arrays); exp/log/rcp by the hardware ops
+  // FP64 arithmetic with FP32 cores for exp and ln: e^x = 2^n · 2^f with the FP64 reduction x log2 e = n + f (|f| ≤ ½)
+  // and v_exp_f32 for 2^f; ln x = (e + log2 m) ln 2 with the FP64 exponent/mantissa split and v_log_f32 for log2 m
+  // (relative error of a few FP32 ulps over the whole FP64 range).
+  bool fast_transcendentals = false;
 };
 // Straight-line kernel: signature (const double* const* in, double* const* out, unsigned n, unsigned offset).
 std::string emit_straight(const ast::Linear& l, const std::string& name, const StraightOptions& o);

@@ -40,33 +40,36 @@ class Straight : public SoAForm {
 }  // namespace
 
 // Form 0 keeps every operation as the FP64 interpreter rounds it; form 1 on a straight-line program allows FMA
-// contraction and reciprocal-multiply division (both within an ulp or two per operation).
+// contraction and reciprocal-multiply division (both within an ulp or two per operation); form 3 also evaluates
+// exp and ln with FP32 cores.
 StraightOptions straight_options(int form, unsigned block_size) {
   StraightOptions o;
   o.block_size = block_size ? block_size : 256;
-  o.reciprocal = form == 1;
+  o.reciprocal = form == 1 || form == 3;
+  o.fast_transcendentals = form == 3;
   return o;
 }
 
-// Chemistry programs (without the reactor right-hand side) take the tiled kernels in forms 1 and 2.
-bool use_tiled(const ProgramView& p, int form) { return form == 2 || (form == 1 && tiled_applies(p)); }
+// Chemistry programs (without the reactor right-hand side) take the per-section kernels in forms 1-3; form 2 needs
+// them.
+bool use_tiled(const ProgramView& p, int form) { return form == 2 || ((form == 1 || form == 3) && tiled_applies(p)); }
 
 std::string form_source(const ProgramView& p, int form, unsigned block_size) {
-  if (form < 0 || form > 2) throw ArgumentError("unknown form");
-  if (use_tiled(p, form)) return tiled_source(p, form == 2, block_size);
+  if (form < 0 || form > 3) throw ArgumentError("unknown form");
+  if (use_tiled(p, form)) return tiled_source(p, form, block_size);
   return emit_straight(*p.linear, "program", straight_options(form, block_size));
 }
 
 void compile_form(const ProgramView& p, int form, unsigned block_size) {
-  if (form < 0 || form > 2) throw ArgumentError("unknown form");
-  if (use_tiled(p, form)) return compile_tiled(p, form == 2, block_size);
-  device::compile(emit_straight(*p.linear, "program", straight_options(form, block_size)), "program", hip_options(form == 1));
+  if (form < 0 || form > 3) throw ArgumentError("unknown form");
+  if (use_tiled(p, form)) return compile_tiled(p, form, block_size);
+  device::compile(emit_straight(*p.linear, "program", straight_options(form, block_size)), "program", hip_options(form != 0));
 }
 
 std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size) {
-  if (form < 0 || form > 2) throw ArgumentError("unknown form");
-  if (use_tiled(p, form)) return assemble_tiled(p, form == 2, block_size);
-  return std::make_unique<Straight>(p, straight_options(form, block_size), form == 1);
+  if (form < 0 || form > 3) throw ArgumentError("unknown form");
+  if (use_tiled(p, form)) return assemble_tiled(p, form, block_size);
+  return std::make_unique<Straight>(p, straight_options(form, block_size), form != 0);
 }
 
 }  // namespace combustion

@@ -57,9 +57,10 @@ std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned blo
 
 // Tiled forms (tiled.cpp).
 bool tiled_applies(const ProgramView& p);
-std::string tiled_source(const ProgramView& p, bool mixed, unsigned block_size);
-void compile_tiled(const ProgramView& p, bool mixed, unsigned block_size);
-std::unique_ptr<Form> assemble_tiled(const ProgramView& p, bool mixed, unsigned block_size);
+// form: 1 FP64, 2 MFMA mixed precision, 3 FP32-core straight-line rates (see cb_form).
+std::string tiled_source(const ProgramView& p, int form, unsigned block_size);
+void compile_tiled(const ProgramView& p, int form, unsigned block_size);
+std::unique_ptr<Form> assemble_tiled(const ProgramView& p, int form, unsigned block_size);
 
 // Transport surrogate (kernels/surrogate.hip): network parameters in the Mlp layout for n_in = n_out = K+2 and
 // the per-feature and per-output normalisation, training box included.

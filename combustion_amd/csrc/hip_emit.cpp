@@ -29,6 +29,18 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
          "  e = __builtin_fma(-b, r, 1.0);\n"
          "  return __builtin_fma(r, e, r);\n"
          "}\n";
+  if (o.fast_transcendentals && !o.f32)
+    s << "__device__ __forceinline__ double exp_wide(double x) {\n"
+         "  const double y = x * 1.4426950408889634;\n"
+         "  const double n = __builtin_fmin(__builtin_fmax(__builtin_rint(y), -1100.0), 1100.0);\n"
+         "  return __builtin_ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);\n"
+         "}\n"
+         "__device__ __forceinline__ double log_wide(double x) {\n"
+         "  const long long b = __double_as_longlong(x);\n"
+         "  const int e = (int)((b >> 52) & 0x7FF) - 1023;\n"
+         "  const double m = __longlong_as_double((b & 0x000FFFFFFFFFFFFFLL) | 0x3FF0000000000000LL);\n"
+         "  return ((double)e + (double)__builtin_amdgcn_logf((float)m)) * 0.6931471805599453;\n"
+         "}\n";
   const char* F = o.f32 ? "float" : "double";
   if (o.f32)
     s << "__device__ __forceinline__ float rcp32(float b) { return __builtin_amdgcn_rcpf(b); }\n"
@@ -71,8 +83,8 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
           s << a << " / " << b;
         break;
       case LOp::Sqrt: s << (o.f32 ? "__builtin_sqrtf(" : "sqrt(") << a << ")"; break;
-      case LOp::Exp: s << (o.f32 ? "exp32(" : "exp(") << a << ")"; break;
-      case LOp::Ln: s << (o.f32 ? "log32(" : "log(") << a << ")"; break;
+      case LOp::Exp: s << (o.f32 ? "exp32(" : o.fast_transcendentals ? "exp_wide(" : "exp(") << a << ")"; break;
+      case LOp::Ln: s << (o.f32 ? "log32(" : o.fast_transcendentals ? "log_wide(" : "log(") << a << ")"; break;
       case LOp::Sq: s << a << " * " << a; break;
       case LOp::Select: s << a << " ? " << b << " : " << c; break;
       default: break;

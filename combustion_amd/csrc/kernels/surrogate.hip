@@ -1,10 +1,14 @@
 // Transport surrogate inference for gfx950 (hand-written; the host prepends the sizes).
 //
-// A block of 4 waves evaluates NB = 128 states. Per state: features f = ((T, ln P, Y_0..Y_{K-1}) - mean)/std, zero
-// padded to IN; hidden layers h = SiLU(W h + b) of WIDTH; output r = W h + b of OUT (first K+2 valid). Outputs are
-// [λ, μ, D_0..D_{K-1}] = base · exp(r·σ + m): the network predicts the log of each property relative to a cheap
-// baseline (the bulk species' conductivity and viscosity fits and the binary diffusion coefficients against the
-// bulk species, transport.rs:125-136 fits, D_kb = d_kb(ℓ) T^1.5 / P), standardised by (m, σ).
+// A block of 4 waves evaluates NB = 128 states. Per state, from the inputs (T, P, Y_0..Y_{K-1}):
+//   mole fractions X_k = (Y_k/W_k) / Σ_j Y_j/W_j and the O(K) baselines, from the transport fits in ℓ = ln(T/T0)
+//   (transport.rs:125-136; λ_k = c_k(ℓ)√T, η_k = v_k(ℓ)²√T, D_kb = d_kb(ℓ) T^1.5/P against the bulk species b):
+//     λ_b = ½ (Σ X_k λ_k + 1/Σ X_k/λ_k)   the mixture rule of transport.rs:141-146 itself, which is O(K)
+//     μ_b = Σ X_k η_k                       linear mixing instead of Wilke's O(K²) sum
+//     D_b,k = D_kb (1 - Y_k)/(1 - X_k)      the mixture-averaged rule with every D_kj replaced by D_kb (b' = 0 for k = b)
+//   features f = ((T, ln P, X_0..X_{K-1}) - mean)/std, zero padded to IN;
+//   hidden layers h = SiLU(W h + b) of WIDTH; output r = W h + b of OUT (first K+2 valid);
+//   [λ, μ, D_0..D_{K-1}] = baseline · exp(r σ + m), with (m, σ) the training set's standardisation.
 // Layers run on v_mfma_f32_16x16x16bf16_1k: A = W rows (weights, BF16, read from memory), B = activations
 // (features × states, BF16 in LDS), FP32 accumulators; wave w owns a quarter of the output features of every layer
 // for all NB states, and the block rewrites the activation buffer in place after a barrier.
@@ -29,10 +33,15 @@ __device__ __forceinline__ u16 to_bf16(float x) {
 __device__ __forceinline__ float silu(float z) {
   return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
 }
+__device__ __forceinline__ float poly(const float* __restrict__ c, float l) {
+  float p = c[DK - 1];
+  for (int j = DK - 2; j >= 0; j--) p = p * l + c[j];
+  return p;
+}
 
 struct Lds {
   u16 h[NB][HSTRIDE];
-  float lnT[NB], lnP[NB], l[NB];
+  float lnT[NB], lnP[NB], l[NB], rN[NB], lam[NB], mu[NB];
   int inside[NB];
 };
 
@@ -58,33 +67,46 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
                     const u16* __restrict__ w16, const float* __restrict__ params, const long* __restrict__ offsets,
                     const float* __restrict__ f_mean, const float* __restrict__ f_rstd, const float* __restrict__ lo,
                     const float* __restrict__ hi, const float* __restrict__ o_mean, const float* __restrict__ o_std,
-                    const float* __restrict__ base_c, const float* __restrict__ base_v, const float* __restrict__ base_d,
-                    float T0, unsigned* __restrict__ fallback, unsigned* __restrict__ fallback_count) {
+                    const float* __restrict__ fit_c, const float* __restrict__ fit_v, const float* __restrict__ fit_db,
+                    const float* __restrict__ rcp_W, float T0, unsigned* __restrict__ fallback,
+                    unsigned* __restrict__ fallback_count) {
   __shared__ Lds L;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const unsigned s0 = blockIdx.x * NB;
 
-  // 1. Features (state = tid % NB, features split over the two halves of the block) and the box test.
+  // 1. Per state (one thread each): mole fractions, baselines, features and the box test.
   if (tid < NB) {
-    const unsigned s = s0 + tid;
-    const bool valid = s < n;
-    const double T = valid ? in[0][s] : 1000.0, P = valid ? in[1][s] : 101325.0;
-    L.lnT[tid] = (float)log(T);
-    L.lnP[tid] = (float)log(P);
-    L.l[tid] = (float)log(T / (double)T0);
-    L.inside[tid] = valid;
-  }
-  __syncthreads();
-  for (int idx = tid; idx < NB * IN; idx += 256) {
-    const int st = idx % NB, f = idx / NB;
+    const int st = tid;
     const unsigned s = s0 + st;
-    float v = 0.f;
-    if (f < NF && s < n) {
-      const float x = f == 0 ? (float)in[0][s] : f == 1 ? L.lnP[st] : (float)in[f][s];
-      if (!(x >= lo[f] && x <= hi[f])) L.inside[st] = 0;  // benign race: every writer stores 0
-      v = (x - f_mean[f]) * f_rstd[f];
+    const bool valid = s < n;
+    int inside = valid;
+    const double Td = valid ? in[0][s] : 1000.0, Pd = valid ? in[1][s] : 101325.0;
+    const float T = (float)Td, lnP = (float)log(Pd), l = (float)log(Td / (double)T0);
+    float N = 0.f;
+    for (int k = 0; k < K; k++) N += (valid ? (float)in[2 + k][s] : 1.f / K) * rcp_W[k];
+    const float rN = 1.f / N, sqrtT = __builtin_sqrtf(T);
+    float a = 0.f, b = 0.f, m = 0.f;
+    for (int k = 0; k < K; k++) {
+      const float X = (valid ? (float)in[2 + k][s] : 1.f / K) * rcp_W[k] * rN;
+      const float lam = poly(fit_c + k * DK, l) * sqrtT, v = poly(fit_v + k * DK, l);
+      a += X * lam;
+      b += X * __builtin_amdgcn_rcpf(lam);
+      m += X * v * v;
+      const int f = 2 + k;
+      if (!(X >= lo[f] && X <= hi[f])) inside = 0;
+      L.h[st][f] = to_bf16((X - f_mean[f]) * f_rstd[f]);
     }
-    L.h[st][f] = to_bf16(v);
+    if (!(T >= lo[0] && T <= hi[0]) || !(lnP >= lo[1] && lnP <= hi[1])) inside = 0;
+    L.h[st][0] = to_bf16((T - f_mean[0]) * f_rstd[0]);
+    L.h[st][1] = to_bf16((lnP - f_mean[1]) * f_rstd[1]);
+    for (int f = NF; f < IN; f++) L.h[st][f] = 0;
+    L.lnT[st] = __builtin_logf(T);
+    L.lnP[st] = lnP;
+    L.l[st] = l;
+    L.rN[st] = rN;
+    L.lam[st] = 0.5f * (a + __builtin_amdgcn_rcpf(b));
+    L.mu[st] = m * sqrtT;
+    L.inside[st] = inside;
   }
   __syncthreads();
 
@@ -121,17 +143,20 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
       const int st = 16 * c + (lane & 15);
       const unsigned s = s0 + st;
       if (s >= n || !L.inside[st]) continue;
-      const float l = L.l[st], lnT = L.lnT[st], lnP = L.lnP[st];
       for (int i = 0; i < 4; i++) {
         const int o = wave * (OUT / 4) + 16 * r + 4 * (lane >> 4) + i;
         if (o >= NF) continue;
-        const float* cf = o == 0 ? base_c : o == 1 ? base_v : base_d + (size_t)(o - 2) * DK;
-        float p = cf[DK - 1];
-        for (int j = DK - 2; j >= 0; j--) p = p * l + cf[j];
-        // ln base: λ = c √T, μ = v² √T, D = d T^1.5 / P
-        const float lnbase = o == 0 ? __builtin_logf(p) + 0.5f * lnT
-                           : o == 1 ? 2.f * __builtin_logf(p) + 0.5f * lnT
-                                    : __builtin_logf(p) + 1.5f * lnT - lnP;
+        float lnbase;
+        if (o == 0) {
+          lnbase = __builtin_logf(L.lam[st]);
+        } else if (o == 1) {
+          lnbase = __builtin_logf(L.mu[st]);
+        } else {
+          const int k = o - 2;
+          const float Yk = (float)in[2 + k][s], Xk = Yk * rcp_W[k] * L.rN[st];
+          lnbase = __builtin_logf(poly(fit_db + k * DK, L.l[st])) + 1.5f * L.lnT[st] - L.lnP[st] +
+                   __builtin_logf(fmaxf(1.f - Yk, 1e-30f)) - __builtin_logf(fmaxf(1.f - Xk, 1e-30f));
+        }
         const float y = (acc[r][c][i] + params[b_off + o]) * o_std[o] + o_mean[o] + lnbase;
         out[o][s] = (double)__builtin_expf(y);
       }

@@ -1,4 +1,5 @@
-// Mixture-averaged transport for gfx950 (hand-written; the host prepends the sizes and the precision).
+// Mixture-averaged transport for gfx950 (hand-written; the host prepends the sizes and the precision), for
+// mechanisms too large for the straight-line transport program (K > 64) and for lists of states.
 //
 // A block of WAVES waves owns NS = 16 states; species, Wilke row tiles and diffusion tiles are split over the
 // waves. With x_k = C_k the concentrations and the fits in ℓ = ln(T/T0) (transport.rs:125-136):

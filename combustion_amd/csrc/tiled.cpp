@@ -49,57 +49,74 @@ int model_rank(ReactionModel m) {
   return 0;
 }
 
+constexpr double kLog2e = 1.4426950408889634;
+
 MixedTables mixed_tables(const Mechanism& m) {
   MixedTables t;
   const Species& sp = m.species;
   t.K = (int)sp.len();
   t.KA = (int)m.active;
   const int R = (int)m.reactions.size();
-  t.KB = (t.K + 15) / 16;
+  t.KB = (t.K + 31) / 32;
   t.AB = (t.KA + 15) / 16;
-  t.RT = (R + 15) / 16;
-  // Reactions sorted by model so that most 16-reaction tiles hold one model.
+  t.KE = (t.K + 3) / 4;
+  // Rows: the reactions sorted by model, each model group padded to whole 16-row halves so that every half holds one
+  // model (the kernel then branches on the half, not on the row); then padded to whole 32-row tiles.
   std::vector<int> order(R);
   for (int r = 0; r < R; r++) order[r] = r;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return model_rank(m.reactions[a].model) < model_rank(m.reactions[b].model); });
-  const int RP = t.RT * 16;
-  t.A_fwd.assign((size_t)t.RT * t.KB * 64 * 4, 0);
-  t.A_rev.assign((size_t)t.RT * t.KB * 64 * 4, 0);
-  t.A_nu.assign((size_t)t.RT * t.AB * 64 * 4, 0);
-  // A operand of v_mfma_f32_16x16x16_bf16: lane L holds A[row L%16][col 4 (L/16) + i].
+  std::vector<int> rows;  // padded row -> reaction (or -1)
+  for (size_t i = 0; i < order.size();) {
+    const int rank = model_rank(m.reactions[order[i]].model);
+    while (i < order.size() && model_rank(m.reactions[order[i]].model) == rank) rows.push_back(order[i++]);
+    while (rows.size() % 16) rows.push_back(-1);
+  }
+  while (rows.size() % 32 || rows.empty()) rows.push_back(-1);
+  t.RT = (int)rows.size() / 32;
+  const int HT = 2 * t.RT, RP = 32 * t.RT;
+  // A operands of v_mfma_f32_16x16x32_bf16: lane L, element j is A[row L%16][k = 8 (L/16) + j].
+  t.A_fwd.assign((size_t)HT * t.KB * 64 * 8, 0);
+  t.A_rev.assign((size_t)HT * t.KB * 64 * 8, 0);
+  for (int ht = 0; ht < HT; ht++)
+    for (int kb = 0; kb < t.KB; kb++)
+      for (int L = 0; L < 64; L++)
+        for (int j = 0; j < 8; j++) {
+          const int r = 16 * ht + L % 16, k = 32 * kb + 8 * (L / 16) + j;
+          if (rows[r] < 0 || k >= t.K) continue;
+          const Reaction& x = m.reactions[rows[r]];
+          const size_t at = (((size_t)ht * t.KB + kb) * 64 + L) * 8 + j;
+          t.A_fwd[at] = bf16_bits((float)x.reactants[k]);
+          t.A_rev[at] = bf16_bits((float)x.products[k]);
+        }
+  // ν·q: A[species row][reaction k] with k of (L/16, j) = j < 4 ? 4 (L/16) + j : 16 + 4 (L/16) + j - 4, the rows the
+  // lane's two 16-reaction halves produce.
+  t.A_nu.assign((size_t)t.RT * t.AB * 64 * 8, 0);
   for (int rt = 0; rt < t.RT; rt++)
-    for (int L = 0; L < 64; L++)
-      for (int i = 0; i < 4; i++) {
-        const int row = L % 16, col = 4 * (L / 16) + i;
-        for (int kb = 0; kb < t.KB; kb++) {
-          const int r = rt * 16 + row, k = kb * 16 + col;
-          if (r < R && k < t.K) {
-            const Reaction& x = m.reactions[order[r]];
-            const size_t at = (((size_t)rt * t.KB + kb) * 64 + L) * 4 + i;
-            t.A_fwd[at] = bf16_bits((float)x.reactants[k]);
-            t.A_rev[at] = bf16_bits((float)x.products[k]);
-          }
+    for (int a = 0; a < t.AB; a++)
+      for (int L = 0; L < 64; L++)
+        for (int j = 0; j < 8; j++) {
+          const int g = L / 16, k = 16 * a + L % 16;
+          const int r = 32 * rt + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+          if (rows[r] >= 0 && k < t.KA)
+            t.A_nu[(((size_t)rt * t.AB + a) * 64 + L) * 8 + j] = bf16_bits((float)m.reactions[rows[r]].net[k]);
         }
-        for (int a = 0; a < t.AB; a++) {
-          const int k = a * 16 + row, r = rt * 16 + col;
-          if (r < R && k < t.KA)
-            t.A_nu[(((size_t)rt * t.AB + a) * 64 + L) * 4 + i] = bf16_bits((float)m.reactions[order[r]].net[k]);
-        }
-      }
-  t.KE = (t.K + 3) / 4;
   t.rate.assign(4 * RP, 0.);
   t.k0.assign(4 * RP, 0.);
   t.troe.assign(4 * RP, 0.f);
-  const int zero_row = t.KA * 16 * 8;  // byte offset of the zero row of the Gibbs table (rows of 16 doubles)
-  t.slots.assign(8 * RP, zero_row);
+  // G tables: row k of the plus table at byte k·128, of the minus table at (KA+1+k)·128; row KA (zero) pads.
+  const int row_bytes = 16 * 8, zero = t.KA * row_bytes;
+  const int minus0 = (t.KA + 1) * row_bytes;
+  if (2 * (t.KA + 1) * row_bytes > 65535) throw Unsupported("too many species for the mixed kernel's 16-bit slots");
+  t.slots.assign(4 * RP, 0);
   for (int r = 0; r < RP; r++) {
-    t.slots[8 * r] = 5 | (8 << 8);
-    t.slots[8 * r + 7] = 0;
+    t.slots[4 * r] = 5;
+    for (int w = 1; w < 4; w++) t.slots[4 * r + w] = zero | (zero << 16);
   }
-  t.tile_info.assign(t.RT, 0);
-  t.eff.assign((size_t)t.RT * t.KE * 64, 0.f);
-  for (int r = 0; r < R; r++) {
-    const Reaction& x = m.reactions[order[r]];
+  t.tile_info.assign(HT, 5);  // 5: a half of padding rows only
+  t.eff.assign((size_t)HT * t.KE * 64, 0.f);
+  for (int r = 0; r < RP; r++) {
+    if (rows[r] < 0) continue;
+    const Reaction& x = m.reactions[rows[r]];
     const RateConstant& k = x.rate_constant;
     int type = 0;
     switch (x.model) {
@@ -110,15 +127,16 @@ MixedTables mixed_tables(const Mechanism& m) {
       case ReactionModel::Troe: type = 4; break;
       default: throw Unsupported("reaction model not supported by the mixed form");
     }
-    t.rate[4 * r] = std::log(k.preexponential_factor);
+    t.rate[4 * r] = std::log2(k.preexponential_factor);
     t.rate[4 * r + 1] = k.temperature_exponent;
-    t.rate[4 * r + 2] = k.activation_temperature;
+    t.rate[4 * r + 2] = k.activation_temperature * kLog2e;
+    t.rate[4 * r + 3] = -x.sum_net;
     int kinf_const = 0;
     if (type >= 3) {
       // Troe factors A∞ out of k0 (reaction.rs:163-170)
-      t.k0[4 * r] = std::log(type == 4 ? x.k0.preexponential_factor / k.preexponential_factor : x.k0.preexponential_factor);
+      t.k0[4 * r] = std::log2(type == 4 ? x.k0.preexponential_factor / k.preexponential_factor : x.k0.preexponential_factor);
       t.k0[4 * r + 1] = x.k0.temperature_exponent;
-      t.k0[4 * r + 2] = x.k0.activation_temperature;
+      t.k0[4 * r + 2] = x.k0.activation_temperature * kLog2e;
     }
     if (type == 4) {
       kinf_const = k.temperature_exponent == 0. && k.activation_temperature == 0.;
@@ -127,33 +145,25 @@ MixedTables mixed_tables(const Mechanism& m) {
       t.troe[4 * r + 2] = x.troe.T1 < 1e30 ? (float)(1. / x.troe.T1) : 0.f;
       t.troe[4 * r + 3] = std::isfinite(x.troe.T2) ? (float)x.troe.T2 : 0.f;
     }
-    if (-x.sum_net < -8 || -x.sum_net > 7) throw Unsupported("change in moles out of range: " + x.equation);
-    t.slots[8 * r] = type | (kinf_const << 4) | ((8 - x.sum_net) << 8);
-    int plus = 0, minus = 0;
+    t.slots[4 * r] = type | (kinf_const << 4);
+    std::vector<int> offsets;
     if (type != 1)
       for (size_t s = 0; s < x.net.size(); s++) {
         const int nu = x.net[s];
-        if (!nu) continue;
-        if (std::abs(nu) > 2) throw Unsupported("|ν| > 2 in reaction " + x.equation);
-        const int slot = (int)s * 16 * 8 | (std::abs(nu) == 2);
-        if (nu > 0) {
-          if (plus == 3) throw Unsupported("more than 3 products change in reaction " + x.equation);
-          t.slots[8 * r + 1 + plus++] = slot;
-        } else {
-          if (minus == 3) throw Unsupported("more than 3 reactants change in reaction " + x.equation);
-          t.slots[8 * r + 4 + minus++] = slot;
-        }
+        for (int c = 0; c < std::abs(nu); c++) offsets.push_back(nu > 0 ? (int)s * row_bytes : minus0 + (int)s * row_bytes);
       }
-    const int rt = r / 16;
-    if (type >= 2) t.tile_info[rt] |= 1;
-    if (type >= 3) t.tile_info[rt] |= 2;
-    if (type == 4) t.tile_info[rt] |= 4;
+    if (offsets.size() > 6) throw Unsupported("more than 6 stoichiometric changes in reaction " + x.equation);
+    while (offsets.size() < 6) offsets.push_back(zero);
+    for (int w = 0; w < 3; w++) t.slots[4 * r + 1 + w] = offsets[2 * w] | (offsets[2 * w + 1] << 16);
+    // the half's model: elementary and irreversible rows share halves (type 0), the others have halves of their own
+    const int ht = r / 16;
+    t.tile_info[ht] = type <= 1 ? 0 : type;
     // A operand of v_mfma_f32_16x16x4_f32: lane L holds A[row L%16][col L/16] = e - 1 of species 4 kk + L/16.
     if (type >= 2)
       for (int kk = 0; kk < t.KE; kk++)
         for (int L = 0; L < 64; L++) {
           const int row = L % 16, s = kk * 4 + L / 16;
-          if (rt * 16 + row == r && s < t.K) t.eff[((size_t)rt * t.KE + kk) * 64 + L] = (float)(x.efficiencies[s] - 1.);
+          if (ht * 16 + row == r && s < t.K) t.eff[((size_t)ht * t.KE + kk) * 64 + L] = (float)(x.efficiencies[s] - 1.);
         }
   }
   for (int k = 0; k < t.K; k++) t.rcp_W.push_back(1. / sp.molar_mass[k]);
@@ -167,7 +177,8 @@ MixedTables mixed_tables(const Mechanism& m) {
     t.split.push_back(std::isnan(n.temperature_split) ? INFINITY : n.temperature_split);
   }
   // LDS per wave (struct Lds of rates_mixed.hip): as many waves per block (up to 4) as fit in 160 KiB.
-  const size_t per_wave = (size_t)t.KB * 3 * 64 * 4 * 2 + (size_t)(t.KA + 1) * 16 * 8 + (size_t)t.KE * 4 * 16 * 4 + 16 * (3 * 8 + 2 * 4);
+  const size_t staging = (size_t)t.KB * 3 * 64 * 8 * 2, gtables = (size_t)2 * (t.KA + 1) * 16 * 8;
+  const size_t per_wave = std::max(staging, gtables) + (size_t)t.KE * 4 * 16 * 4 + 16 * (3 * 8 + 2 * 4);
   t.WAVES = (int)std::min<size_t>(4, 163840 / per_wave);
   if (t.WAVES < 1) throw Unsupported("mechanism too large for the mixed rates kernel's LDS");
   t.ln_clamp_pos = std::log(6981463048192.);  // reaction.rs:244-256
@@ -191,13 +202,13 @@ std::string rates_mixed_text(const MixedTables& t) {
   return s.str();
 }
 
-// Tables of kernels/transport_tiled.hip: unscaled fits (transport.rs:125-136) and the Wilke and diffusion
-// operands in the lane layouts of v_mfma_{f64,f32}_16x16x4 (A[row L%16][col L/16]).
+// Tables of kernels/transport_tiled.hip: unscaled fits (transport.rs:125-136), W^1/4, the Wilke matrix a² and the
+// diffusion-fit tiles in the lane layout of v_mfma_{f64,f32}_16x16x4 (A[row L%16][col L/16]).
 struct TransportTables {
   int K, KP, DK, NTILE;
   double T0;
   std::vector<double> rcp_W, W, fit_c, fit_v, wilke, dcoef;
-  std::vector<int32_t> tiles;
+  std::vector<int32_t> tiles;  // (k, j0) per diffusion tile
 };
 
 TransportTables transport_tables(const Mechanism& m, const Polynomials& p, double T0) {
@@ -295,7 +306,7 @@ class TransportTiled {
     void* args[] = {&in, &out, &n32, &d_rcp_W_, &d_W_, &d_c_, &d_v_, &d_wilke_, &d_dcoef_, &d_tiles_, &T0, &idx, &count};
     const unsigned states_per_block = 16;  // 4 waves share 16 states
     unsigned grid = (unsigned)((n + states_per_block - 1) / states_per_block);
-    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch transport_tiled");
+    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch transport");
   }
 
  private:
@@ -313,12 +324,13 @@ void check_program(const ProgramView& p) {
 }
 
 // The rates section of a chemistry program as straight-line FP64 code (contraction and reciprocal division).
-std::string rates_fp64_text(const ProgramView& p) {
+std::string rates_straight_text(const ProgramView& p, bool fast) {
   ChemistryOptions o;
   o.transport = false;
   StraightOptions so;
   so.reciprocal = true;
-  return emit_straight(ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o)), "rates_fp64", so);
+  so.fast_transcendentals = fast;
+  return emit_straight(ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o)), fast ? "rates_fast" : "rates_fp64", so);
 }
 
 // rates_mixed with its tables in device memory; writes ω to out[0..Ka).
@@ -364,11 +376,12 @@ class RatesMixed {
   double *d_rcp_W_ = nullptr, *d_gibbs_ = nullptr, *d_split_ = nullptr;
 };
 
-// The rates section as straight-line FP64 code; writes ω to out[0..Ka).
-class RatesFp64 {
+// The rates section as straight-line FP64 code (with FP32 cores for exp and ln when fast); writes ω to out[0..Ka).
+class RatesStraight {
  public:
-  explicit RatesFp64(const ProgramView& p) {
-    kernel_.load(device::compile(rates_fp64_text(p), "rates_fp64", hip_options(true)), "rates_fp64");
+  RatesStraight(const ProgramView& p, bool fast) {
+    const char* name = fast ? "rates_fast" : "rates_fp64";
+    kernel_.load(device::compile(rates_straight_text(p, fast), name, hip_options(true)), name);
   }
   void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
     unsigned n32 = (unsigned)n, offset = 0, block = 256;
@@ -381,31 +394,72 @@ class RatesFp64 {
   device::Kernel kernel_;
 };
 
+// Mechanisms up to 64 species take the transport section as straight-line code (one lane per state): at K=53 it
+// issues fewer instructions per state than the tiled kernel, whose MFMA contractions do not pay for the
+// reciprocals and reductions around them. Larger mechanisms (a K² program) take transport_tiled.
+bool straight_transport(const ProgramView& p) { return p.mechanism->species.len() <= 64; }
+
+std::string transport_straight_text(const ProgramView& p, bool f64) {
+  ChemistryOptions o;
+  o.rates = false;
+  StraightOptions so;
+  so.reciprocal = f64;
+  so.f32 = !f64;
+  return emit_straight(ast::linearize(chemistry(*p.mechanism, p.chemistry->polynomials.get(), p.chemistry->scales, o)),
+                       f64 ? "transport_fp64" : "transport_f32", so);
+}
+
+class TransportStraight {
+ public:
+  TransportStraight(const ProgramView& p, bool f64) {
+    const char* name = f64 ? "transport_fp64" : "transport_f32";
+    kernel_.load(device::compile(transport_straight_text(p, f64), name, hip_options(true)), name);
+  }
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
+    unsigned n32 = (unsigned)n, offset = 0, block = 256;
+    void* args[] = {&in, &out, &n32, &offset};
+    unsigned grid = (unsigned)((n + block - 1) / block);
+    device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, block, 1, 1, 0, stream, args, nullptr), "launch transport");
+  }
+
+ private:
+  device::Kernel kernel_;
+};
+
 class Tiled : public SoAForm {
  public:
-  Tiled(const ProgramView& p, bool mixed) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
+  // form 1: FP64 rates and transport; 2: MFMA mixed rates, FP32 transport; 3: FP32-core rates, FP32 transport.
+  Tiled(const ProgramView& p, int form) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
     check_program(p);
     const auto& o = p.chemistry->options;
+    const bool f64 = form == 1;
     if (o.rates) {
       n_rates_ = (int)p.mechanism->active;
-      if (mixed)
+      if (form == 2)
         mixed_ = std::make_unique<RatesMixed>(*p.mechanism);
       else
-        fp64_ = std::make_unique<RatesFp64>(p);
+        fp64_ = std::make_unique<RatesStraight>(p, form == 3);
     }
-    if (o.transport) transport_ = std::make_unique<TransportTiled>(p, !mixed);
+    if (o.transport) {
+      if (straight_transport(p))
+        straight_ = std::make_unique<TransportStraight>(p, f64);
+      else
+        transport_ = std::make_unique<TransportTiled>(p, f64);
+    }
   }
 
  protected:
   void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
     if (mixed_) mixed_->launch(in, out, n, stream);
     if (fp64_) fp64_->launch(in, out, n, stream);
+    if (straight_) straight_->launch(in, out + n_rates_, n, stream);
     if (transport_) transport_->launch(in, out + n_rates_, n, stream);
   }
 
  private:
   std::unique_ptr<RatesMixed> mixed_;
-  std::unique_ptr<RatesFp64> fp64_;
+  std::unique_ptr<RatesStraight> fp64_;
+  std::unique_ptr<TransportStraight> straight_;
   std::unique_ptr<TransportTiled> transport_;
   int n_rates_ = 0;
 };
@@ -424,7 +478,8 @@ std::string surrogate_text(int K, const MlpLayout& lay, int DK) {
   return s.str();
 }
 
-// Mixed rates, then the transport surrogate with the FP32 transport kernel on the states outside its box.
+// FP32-core straight-line rates, then the transport surrogate with the FP32 transport kernel on the states outside
+// its box.
 class Surrogate : public SoAForm {
  public:
   Surrogate(const ProgramView& p, const SurrogateWeights& w) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
@@ -439,23 +494,24 @@ class Surrogate : public SoAForm {
       if ((int)v->size() != NF) throw ArgumentError("surrogate normalisation arrays need K+2 entries");
     if (o.rates) {
       n_rates_ = (int)m.active;
-      rates_ = std::make_unique<RatesMixed>(m);
+      rates_ = std::make_unique<RatesStraight>(p, true);
     }
     transport_ = std::make_unique<TransportTiled>(p, false);
     const Polynomials& poly = polynomials(p);
     const int D = poly.D, DK = D;
     if (D > 8) throw Unsupported("transport fits longer than 8 coefficients");
-    // Baselines: the bulk species' conductivity and viscosity fits, and d_kb against the bulk species (species 0 for
-    // the bulk species itself).
+    // Baselines (kernels/surrogate.hip): every species' conductivity and viscosity fits, d_kb against the bulk
+    // species b (species 0 for the bulk species itself), 1/W.
     const int b = K - 1;
-    std::vector<float> bc, bv, bd;
-    for (int d = 0; d < D; d++) {
-      bc.push_back((float)poly.conductivityIVT[b * D + d]);
-      bv.push_back((float)poly.VviscosityIVVT[b * D + d]);
-    }
+    std::vector<float> fc, fv, fd, rw;
     for (int k = 0; k < K; k++) {
       const int j = k == b ? 0 : b;
-      for (int d = 0; d < D; d++) bd.push_back((float)poly.diffusivityITVT[((size_t)k * K + j) * D + d]);
+      for (int d = 0; d < D; d++) {
+        fc.push_back((float)poly.conductivityIVT[k * D + d]);
+        fv.push_back((float)poly.VviscosityIVVT[k * D + d]);
+        fd.push_back((float)poly.diffusivityITVT[((size_t)k * K + j) * D + d]);
+      }
+      rw.push_back((float)(1. / m.species.molar_mass[k]));
     }
     std::vector<long> offsets;
     for (const auto& L : lay.layers) {
@@ -472,15 +528,16 @@ class Surrogate : public SoAForm {
     d_hi_ = upload(w.hi);
     d_o_mean_ = upload(w.o_mean);
     d_o_std_ = upload(w.o_std);
-    d_bc_ = upload(bc);
-    d_bv_ = upload(bv);
-    d_bd_ = upload(bd);
+    d_bc_ = upload(fc);
+    d_bv_ = upload(fv);
+    d_bd_ = upload(fd);
+    d_rw_ = upload(rw);
     T0_ = (float)p.chemistry->scales.T0;
     device::check(hipMalloc(&d_count_, sizeof(unsigned)), "hipMalloc");
   }
   ~Surrogate() override {
     for (void* ptr : {(void*)d_w16_, (void*)d_params_, (void*)d_offsets_, (void*)d_f_mean_, (void*)d_f_rstd_, (void*)d_lo_,
-                      (void*)d_hi_, (void*)d_o_mean_, (void*)d_o_std_, (void*)d_bc_, (void*)d_bv_, (void*)d_bd_, (void*)d_count_,
+                      (void*)d_hi_, (void*)d_o_mean_, (void*)d_o_std_, (void*)d_bc_, (void*)d_bv_, (void*)d_bd_, (void*)d_rw_, (void*)d_count_,
                       (void*)d_fallback_})
       if (ptr) (void)hipFree(ptr);
   }
@@ -505,7 +562,7 @@ class Surrogate : public SoAForm {
     device::check(hipMemsetAsync(d_count_, 0, sizeof(unsigned), stream), "hipMemsetAsync");
     unsigned n32 = (unsigned)n;
     void* args[] = {&in, &tout, &n32, &d_w16_, &d_params_, &d_offsets_, &d_f_mean_, &d_f_rstd_, &d_lo_, &d_hi_,
-                    &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_bd_, &T0_, &d_fallback_, &d_count_};
+                    &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_bd_, &d_rw_, &T0_, &d_fallback_, &d_count_};
     const unsigned per_block = 128;
     device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + per_block - 1) / per_block), 1, 1, 256, 1, 1, 0, stream, args,
                                         nullptr),
@@ -514,13 +571,13 @@ class Surrogate : public SoAForm {
   }
 
  private:
-  std::unique_ptr<RatesMixed> rates_;
+  std::unique_ptr<RatesStraight> rates_;
   std::unique_ptr<TransportTiled> transport_;
   int n_rates_ = 0;
   device::Kernel kernel_;
   uint16_t* d_w16_ = nullptr;
   float *d_params_ = nullptr, *d_f_mean_ = nullptr, *d_f_rstd_ = nullptr, *d_lo_ = nullptr, *d_hi_ = nullptr, *d_o_mean_ = nullptr,
-        *d_o_std_ = nullptr, *d_bc_ = nullptr, *d_bv_ = nullptr, *d_bd_ = nullptr;
+        *d_o_std_ = nullptr, *d_bc_ = nullptr, *d_bv_ = nullptr, *d_bd_ = nullptr, *d_rw_ = nullptr;
   long* d_offsets_ = nullptr;
   float T0_ = 1000.f;
   unsigned *d_count_ = nullptr, *d_fallback_ = nullptr;
@@ -531,29 +588,36 @@ class Surrogate : public SoAForm {
 
 bool tiled_applies(const ProgramView& p) { return p.kind == ProgramKind::Chemistry && !p.chemistry->options.rhs; }
 
-std::string tiled_source(const ProgramView& p, bool mixed, unsigned) {
+std::string tiled_source(const ProgramView& p, int form, unsigned) {
   check_program(p);
+  const bool mixed = form != 1;
   std::string s;
-  if (p.chemistry->options.rates) s += mixed ? rates_mixed_text(mixed_tables(*p.mechanism)) : rates_fp64_text(p);
+  if (p.chemistry->options.rates) s += form == 2 ? rates_mixed_text(mixed_tables(*p.mechanism)) : rates_straight_text(p, form == 3);
   if (p.chemistry->options.transport)
-    s += "\n// ---- transport ----\n" + transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed);
+    s += "\n// ---- transport ----\n" + (straight_transport(p) ? transport_straight_text(p, !mixed)
+                                                         : transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed));
   return s;
 }
 
-void compile_tiled(const ProgramView& p, bool mixed, unsigned) {
+void compile_tiled(const ProgramView& p, int form, unsigned) {
   check_program(p);
+  const bool mixed = form != 1;
   if (p.chemistry->options.rates) {
-    if (mixed)
+    if (form == 2)
       device::compile(rates_mixed_text(mixed_tables(*p.mechanism)), "rates_mixed", hip_options(true));
     else
-      device::compile(rates_fp64_text(p), "rates_fp64", hip_options(true));
+      device::compile(rates_straight_text(p, form == 3), form == 3 ? "rates_fast" : "rates_fp64", hip_options(true));
   }
-  if (p.chemistry->options.transport)
-    device::compile(transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed), "transport_tiled",
-                    hip_options(true));
+  if (p.chemistry->options.transport) {
+    if (straight_transport(p))
+      device::compile(transport_straight_text(p, !mixed), mixed ? "transport_f32" : "transport_fp64", hip_options(true));
+    else
+      device::compile(transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed), "transport_tiled",
+                      hip_options(true));
+  }
 }
 
-std::unique_ptr<Form> assemble_tiled(const ProgramView& p, bool mixed, unsigned) { return std::make_unique<Tiled>(p, mixed); }
+std::unique_ptr<Form> assemble_tiled(const ProgramView& p, int form, unsigned) { return std::make_unique<Tiled>(p, form); }
 
 std::unique_ptr<Form> assemble_surrogate(const ProgramView& p, const SurrogateWeights& w) { return std::make_unique<Surrogate>(p, w); }
 

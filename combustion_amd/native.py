@@ -20,8 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcombustion_amd.so")
 
 RATES, TRANSPORT, RHS = 1, 2, 4
-FORM_STRAIGHT, FORM_FP64, FORM_MIXED = 0, 1, 2
-FORMS = {"straight": FORM_STRAIGHT, "fp64": FORM_FP64, "mixed": FORM_MIXED}
+FORM_STRAIGHT, FORM_FP64, FORM_MIXED, FORM_FAST = 0, 1, 2, 3
+FORMS = {"straight": FORM_STRAIGHT, "fp64": FORM_FP64, "mixed": FORM_MIXED, "fast": FORM_FAST}
 
 STATUS = {0: "ok", -1: "argument", -2: "parse", -3: "unsupported", -4: "build", -5: "compile", -6: "device", -7: "size"}
 
@@ -82,6 +82,13 @@ SIGNATURES = {
 def lib():
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7, and whichever copy loads first serves
+        # every later user of that soname. Loading torch first keeps the runtime torch was built against (device
+        # tensors from torch are passed straight to the kernels); without torch the system ROCm runtime is used.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m combustion_amd.build`")
         l = C.CDLL(LIB_PATH)

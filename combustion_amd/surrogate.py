@@ -1,13 +1,16 @@
 """Learned mixture-averaged transport: training data from reactor trajectories, training with the project's
 MLP kernels (kernels/mlp.hip through cb_mlp_*), and the surrogate form (cb_hip_assemble_surrogate).
 
-The network maps the state's features f = (T, ln P, Y_0..Y_{K-1}) to r_o = ln(p_o / b_o) for the K+2 transport
-outputs p = [λ, μ, D_0..D_{K-1}] of the chemistry program (transport.rs:141-177), relative to a baseline b that
-costs O(K) per state: the bulk species' own conductivity c_b(ℓ)√T and viscosity v_b(ℓ)²√T, and the binary
-diffusion coefficient of each species against the bulk species d_kb(ℓ) T^1.5/P (against species 0 for the bulk
-species), all from the transport fits in ℓ = ln(T/T0) (transport.rs:125-136). Features and targets are
-standardised with the training set's means and deviations; the box [min, max] of the training features is the
-domain where the network is used — states outside it are evaluated by the plain transport kernel.
+The network maps the state's features f = (T, ln P, X_0..X_{K-1}) (mole fractions from the mass fractions Y) to
+r_o = ln(p_o / b_o) for the K+2 transport outputs p = [λ, μ, D_0..D_{K-1}] of the chemistry program
+(transport.rs:141-177), relative to baselines b that cost O(K) per state, from the transport fits in ℓ = ln(T/T0)
+(transport.rs:125-136; λ_k = c_k(ℓ)√T, η_k = v_k(ℓ)²√T, D_kb = d_kb(ℓ) T^1.5/P against the bulk species b):
+    λ_b = ½ (Σ X_k λ_k + 1/Σ X_k/λ_k)     the conductivity rule itself, which is O(K)
+    μ_b = Σ X_k η_k                         linear mixing in place of Wilke's O(K²) sum
+    D_b,k = D_kb (1 - Y_k)/(1 - X_k)        every D_kj replaced by D_kb (species 0 partners the bulk species)
+so the network learns what the O(K²) sums add. Features and targets are standardised with the training set's
+means and deviations; the box [min, max] of the training features is the domain where the network is used —
+states outside it are evaluated by the plain transport kernel.
 """
 from __future__ import annotations
 
@@ -38,26 +41,37 @@ class Trained:
         return cls(int(z["width"]), int(z["depth"]), *(z[k] for k in cls.FIELDS))
 
 
-def features(T, P, Y):
-    """[n][K+2] float64: (T, ln P, Y_0..Y_{K-1})."""
-    return np.concatenate([np.asarray(T)[:, None], np.log(np.asarray(P))[:, None], np.asarray(Y).T], axis=1)
+def mole_fractions(mechanism, Y):
+    n = np.asarray(Y) / mechanism.molar_mass[:, None]
+    return n / n.sum(axis=0, keepdims=True)
 
 
-def log_baseline(mechanism, T, P, D=4, T0=1000.0):
+def features(mechanism, T, P, Y):
+    """[n][K+2] float64: (T, ln P, X_0..X_{K-1})."""
+    X = mole_fractions(mechanism, Y)
+    return np.concatenate([np.asarray(T)[:, None], np.log(np.asarray(P))[:, None], X.T], axis=1)
+
+
+def log_baseline(mechanism, T, P, Y, D=4, T0=1000.0):
     """ln b_o [n][K+2] (see the module docstring)."""
     c, v, d = mechanism.transport_fits(D, T0)
     K = mechanism.K
     b = K - 1
     T = np.asarray(T, np.float64)
+    Y = np.asarray(Y, np.float64)
+    X = mole_fractions(mechanism, Y)
     l = np.log(T / T0)
-    pw = np.stack([l ** i for i in range(D)], axis=1)
+    pw = np.stack([l ** i for i in range(D)], axis=1)  # [n][D]
+    lam = (pw @ c.T).T * np.sqrt(T)                      # [K][n]
+    eta = ((pw @ v.T).T) ** 2 * np.sqrt(T)
     out = np.empty((len(T), K + 2))
-    out[:, 0] = np.log(pw @ c[b]) + 0.5 * np.log(T)
-    out[:, 1] = 2 * np.log(pw @ v[b]) + 0.5 * np.log(T)
+    out[:, 0] = np.log(0.5 * ((X * lam).sum(0) + 1 / (X / lam).sum(0)))
+    out[:, 1] = np.log((X * eta).sum(0))
     partner = np.full(K, b)
     partner[b] = 0
     dk = d[np.arange(K), partner]  # [K][D]
-    out[:, 2:] = np.log(pw @ dk.T) + 1.5 * np.log(T)[:, None] - np.log(np.asarray(P))[:, None]
+    Dkb = (pw @ dk.T) * (T ** 1.5 / np.asarray(P))[:, None]  # [n][K]
+    out[:, 2:] = np.log(Dkb) + np.log(np.maximum(1 - Y.T, 1e-30)) - np.log(np.maximum(1 - X.T, 1e-30))
     return out
 
 
@@ -93,8 +107,8 @@ def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384
     import torch
     K = mechanism.K
     NF = K + 2
-    F = features(T, P, Y)
-    R = np.log(np.asarray(props).T) - log_baseline(mechanism, T, P)
+    F = features(mechanism, T, P, Y)
+    R = np.log(np.asarray(props).T) - log_baseline(mechanism, T, P, Y)
     f_mean, f_std = F.mean(0), F.std(0)
     f_rstd = np.where(f_std > 0, 1 / np.maximum(f_std, 1e-300), 0.0)
     o_mean, o_std = R.mean(0), np.maximum(R.std(0), 1e-6)

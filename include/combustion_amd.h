@@ -39,8 +39,10 @@ enum cb_status {
 /* Program forms for cb_hip_assemble. */
 enum cb_form {
   CB_FORM_STRAIGHT = 0, /* the program emitted as straight-line HIP, one lane per state (cuda.rs:101-134) */
-  CB_FORM_FP64 = 1,     /* mechanism-tiled FP64 kernels, constants in constant memory (chemistry programs) */
-  CB_FORM_MIXED = 2     /* tiled mixed precision: BF16-split MFMA contractions, FP32 exp/log, FP64 ΔG/RT */
+  CB_FORM_FP64 = 1,     /* chemistry programs: FP64 rates and transport sections as separate kernels (straight-line rates;
+                           straight-line transport up to 64 species, the tiled MFMA transport kernel above) */
+  CB_FORM_MIXED = 2,    /* tiled mixed precision: BF16-split MFMA contractions, FP32 exp/log, FP64 ΔG/RT */
+  CB_FORM_FAST = 3      /* FP64 straight-line rates with FP32 cores for exp and ln, FP32 transport */
 };
 
 /* Sections of the per-state program built by cb_build_chemistry. */

