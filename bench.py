@@ -3,11 +3,12 @@
 Workload: the 53-species / 325-reaction synthetic mechanism (30 three-body, 25 Troe), 2^20 states in SoA layout
 (T ~ U[600, 2800] K, P ~ U[1, 40] atm, Y ~ Dirichlet(0.3)), outputs ω_k, λ, μ, D_k. A step is one pass of a
 form's kernels over all states, inputs already in HBM. `value` is the plain FP64 form; the line also carries the
-mixed-precision form and the mixed + surrogate form (when a trained network is present) with their speedups and
-their errors against the FP64 oracle on a sample of the same states. Prints one JSON line.
+mixed-precision forms (MFMA contractions; FP32-core straight-line rates) with their speedups and errors against the
+FP64 oracle on a sample of the same states, and — when a trained network is present — the surrogate's workload:
+2^20 held-out reactor-trajectory states, FP64 against FP32-core rates + network transport. Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--states N] [--no-cpu] [--surrogate PATH]
-  python bench.py --only rates|transport|fp64|mixed|copy [--steps K]   (one launch kind alone, for rocprofv3 runs;
+  python bench.py --only rates|transport|fp64|mixed|fast|copy [--steps K]   (one launch kind alone, for rocprofv3 runs;
       copy is a device-to-device copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
 """
 import argparse
@@ -91,6 +92,33 @@ def errors_vs_oracle(out_dev, T, P, Y, n_sample, Ka, rates=True):
     return {name: {"max": float(e.max()), "median": float(np.median(e))} for name, e in sec.items()}
 
 
+def surrogate_workload(args, M, prog, dev):
+    """Config 4: 2^20 held-out states of constant-pressure reactor trajectories (the surrogate's domain), the FP64
+    form against the FP32-core rates + surrogate transport form on the same states."""
+    import torch
+    from combustion_amd import reactor
+    from combustion_amd import surrogate as sur
+    N = args.states
+    t0 = time.perf_counter()
+    T, P, Y = reactor.trajectories(M, 2048, N, seed=4242, max_steps=800, batch=2048)
+    gen_s = time.perf_counter() - t0
+    inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
+    outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
+    fp64 = prog.assemble(form="fp64")
+    wall_p, ms_p = time_module(fp64, inputs, outputs, args.steps, args.warmup)
+    del fp64
+    smod = sur.assemble(prog, sur.Trained.load(args.surrogate))
+    wall_s, ms_s = time_module(smod, inputs, outputs, args.steps, args.warmup)
+    fallback = sur.fallback_count(smod)
+    err = errors_vs_oracle(outputs, T, P, Y, args.error_sample, M.active)
+    return {"states": N, "generation_s": gen_s,
+            "data": "held-out states of 2048 constant-pressure reactor trajectories (seed 4242), T0 ~ U[900,1600] K, "
+                    "P ~ U[1,40] atm, fuel/O2/N2",
+            "fp64": {"states_per_s": N * args.steps / wall_p, "kernel_ms": ms_p},
+            "fast_surrogate": {"states_per_s": N * args.steps / wall_s, "kernel_ms": ms_s, "speedup_vs_fp64": wall_p / wall_s,
+                               "fallback_states": fallback, "error_vs_oracle": err}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,7 +128,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--surrogate", default=SURROGATE)
     ap.add_argument("--error-sample", type=int, default=8192)
-    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "copy"])
+    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "fast", "copy"])
     ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -137,7 +165,8 @@ def main():
         outs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(p.n_outputs)]
         _, ms = time_module(mod, inputs, outs, max(5, args.steps // 2), 2)
         flop = p.op_counts()["weighted"]
-        sections[name] = {"kernel": "rates_fp64" if flags == 1 else "transport_tiled", "kernel_ms": ms,
+        kernel = "rates_fp64" if flags == 1 else ("transport_fp64" if M.K <= 64 else "transport_tiled")
+        sections[name] = {"kernel": kernel, "kernel_ms": ms,
                           "flop_per_state": flop, "achieved_tflops": flop * N / (ms * 1e-3) / 1e12}
         del outs
     dominant = max(sections.values(), key=lambda s: s["kernel_ms"])
@@ -151,7 +180,8 @@ def main():
         "dtype": "f64", "data": "synthetic (seeded mechanism and states)",
         "config": {"workload": "rates+transport, K=53 species / R=325 reactions (30 three-body, 25 Troe), 2^20 states",
                    "states": N, "form": "fp64", "mechanism_seed": synth.CONFIGS[MECHANISM][4]},
-        "roofline": {"bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "compute": "FP64 VALU: straight-line FP64 code, whose peak is the FP64 peak that the "
+                                                 "vector units and the matrix cores share on MI355X", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(dominant["kernel"]),
                      "kernel": dominant["kernel"], "flop_per_state": dominant["flop_per_state"],
                      "kernel_ms": dominant["kernel_ms"],
@@ -163,20 +193,14 @@ def main():
     }
     forms = {"fp64": {"states_per_s": value, "kernel_ms": step_ms}}
 
-    mixed = prog.assemble(form="mixed")
-    wall_m, ms_m = time_module(mixed, inputs, outputs, args.steps, args.warmup)
-    forms["mixed"] = {"states_per_s": N * args.steps / wall_m, "kernel_ms": ms_m, "speedup_vs_fp64": wall / wall_m,
-                      "error_vs_oracle": errors_vs_oracle(outputs, T, P, Y, args.error_sample, Ka)}
+    for form in ("mixed", "fast"):
+        mod = prog.assemble(form=form)
+        wall_f, ms_f = time_module(mod, inputs, outputs, args.steps, args.warmup)
+        forms[form] = {"states_per_s": N * args.steps / wall_f, "kernel_ms": ms_f, "speedup_vs_fp64": wall / wall_f,
+                       "error_vs_oracle": errors_vs_oracle(outputs, T, P, Y, args.error_sample, Ka)}
+        del mod
     if os.path.exists(args.surrogate):
-        trained = sur.Trained.load(args.surrogate)
-        smod = sur.assemble(prog, trained)
-        wall_s, ms_s = time_module(smod, inputs, outputs, args.steps, args.warmup)
-        forms["mixed_surrogate"] = {
-            "states_per_s": N * args.steps / wall_s, "kernel_ms": ms_s, "speedup_vs_fp64": wall / wall_s,
-            "fallback_states": sur.fallback_count(smod),
-            "note": "the benchmark states are drawn over the whole T/P/Y box, not from reactor trajectories, so most "
-                    "of them fall outside the network's training box and take the plain FP32 transport kernel",
-            "error_vs_oracle": errors_vs_oracle(outputs, T, P, Y, args.error_sample, Ka)}
+        result["surrogate_workload"] = surrogate_workload(args, M, prog, dev)
     result["forms"] = forms
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(MECHANISM)
@@ -195,9 +219,9 @@ def only(args, M, inputs, N):
         print(json.dumps({"only": "copy", "bytes_read_per_launch": src.numel() * 4, "bytes_written_per_launch": dst.numel() * 4,
                           "steps": args.steps}))
         return
-    flags = {"rates": 1, "transport": 2, "fp64": 3, "mixed": 3}[args.only]
+    flags = {"rates": 1, "transport": 2, "fp64": 3, "mixed": 3, "fast": 3}[args.only]
     p = M.chemistry(flags)
-    mod = p.assemble(form="mixed" if args.only == "mixed" else "fp64")
+    mod = p.assemble(form=args.only if args.only in ("mixed", "fast") else "fp64")
     outs = [torch.empty(N, dtype=torch.float64, device=inputs[0].device) for _ in range(p.n_outputs)]
     wall, ms = time_module(mod, inputs, outs, args.steps, 1)
     print(json.dumps({"only": args.only, "states": N, "steps": args.steps, "kernel_ms": ms,

@@ -419,6 +419,8 @@ int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w
     copy(sw.hi, w->upper);
     copy(sw.o_mean, w->output_mean);
     copy(sw.o_std, w->output_std);
+    require(w->major && w->n_major >= 1 && w->n_major <= NF - 2, "n_major out of range");
+    sw.major.assign(w->major, w->major + w->n_major);
     ProgramView v{p->mechanism, &p->function, &p->linear, p->kind, &p->chemistry};
     auto* m = new cb_module;
     try {

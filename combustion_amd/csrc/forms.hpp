@@ -67,6 +67,7 @@ std::unique_ptr<Form> assemble_tiled(const ProgramView& p, int form, unsigned bl
 struct SurrogateWeights {
   int width = 256, depth = 3;
   std::vector<float> params, f_mean, f_rstd, lo, hi, o_mean, o_std;
+  std::vector<int32_t> major;  // species whose pair terms the baselines keep
 };
 std::unique_ptr<Form> assemble_surrogate(const ProgramView& p, const SurrogateWeights& w);
 unsigned surrogate_fallback_count(Form& f, hipStream_t stream);

@@ -1,29 +1,41 @@
 // Transport surrogate inference for gfx950 (hand-written; the host prepends the sizes).
 //
-// A block of 4 waves evaluates NB = 128 states. Per state, from the inputs (T, P, Y_0..Y_{K-1}):
-//   mole fractions X_k = (Y_k/W_k) / Σ_j Y_j/W_j and the O(K) baselines, from the transport fits in ℓ = ln(T/T0)
-//   (transport.rs:125-136; λ_k = c_k(ℓ)√T, η_k = v_k(ℓ)²√T, D_kb = d_kb(ℓ) T^1.5/P against the bulk species b):
-//     λ_b = ½ (Σ X_k λ_k + 1/Σ X_k/λ_k)   the mixture rule of transport.rs:141-146 itself, which is O(K)
-//     μ_b = Σ X_k η_k                       linear mixing instead of Wilke's O(K²) sum
-//     D_b,k = D_kb (1 - Y_k)/(1 - X_k)      the mixture-averaged rule with every D_kj replaced by D_kb (b' = 0 for k = b)
+// A block of 4 waves evaluates NB = 64 states. Per state, from the inputs (T, P, Y_0..Y_{K-1}):
+//   mole fractions X_k = (Y_k/W_k) / Σ_j Y_j/W_j and the O(K·NM) baselines, from the transport fits in ℓ = ln(T/T0)
+//   (transport.rs:125-136; λ_k = c_k(ℓ)√T, v_k = v_k(ℓ) with η_k = v_k²√T, D_kj = d_kj(ℓ) T^1.5/P): the mixture rules
+//   with the pair sums restricted to the NM major species S and the rest X_rest = 1 - Σ_S X_j given to the bulk
+//   species b (X'_k = X_rest - [k ∉ S] X_k; r(k) = b, or another major species for k = b):
+//     λ_b = ½ (Σ X_k λ_k + 1/Σ X_k/λ_k)                                 transport.rs:141-146 itself, O(K)
+//     μ_b = √T Σ_k X_k v_k² / den_k,  den_k = Σ_S X_j φ_kj + X'_k φ_kb + [k ∉ S] X_k,  φ_kj = (a_kj + b_kj v_k/v_j)²
+//                                                                        (Wilke, transport.rs:148-158)
+//     D_b,k = T^1.5 (1 - Y_k) / (P S_k),  S_k = Σ_{S, j≠k} X_j/d_kj + X'_k/d_k,r(k)   (transport.rs:162-177)
 //   features f = ((T, ln P, X_0..X_{K-1}) - mean)/std, zero padded to IN;
 //   hidden layers h = SiLU(W h + b) of WIDTH; output r = W h + b of OUT (first K+2 valid);
 //   [λ, μ, D_0..D_{K-1}] = baseline · exp(r σ + m), with (m, σ) the training set's standardisation.
-// Layers run on v_mfma_f32_16x16x16bf16_1k: A = W rows (weights, BF16, read from memory), B = activations
-// (features × states, BF16 in LDS), FP32 accumulators; wave w owns a quarter of the output features of every layer
-// for all NB states, and the block rewrites the activation buffer in place after a barrier.
+// Stage 1 puts one lane per state and splits the species k over the 4 waves (coalesced loads of Y_k; the tables of
+// a species are wave-uniform); the major species' X_j and 1/v_j go through LDS (the activation rows' unused columns)
+// to every wave, and partial sums meet in LDS in wave order. Layers run on v_mfma_f32_16x16x32_bf16: A = 16 weight
+// rows × 32 inputs (16 B per lane from memory, the next step's fragment loaded while the current one multiplies),
+// B = 32 inputs × 16 states (16 B per lane from the activation rows in LDS), FP32 accumulators; wave w owns a quarter of each layer's output
+// rows for all NB states, and the block rewrites the activation rows in place between two barriers. The output
+// layer's values go through LDS so that every output row is stored as 64 consecutive doubles.
 // States whose features leave the training box [lo, hi] are not written: their index goes to fallback[] (count in
 // fallback_count, zeroed by the caller) for the plain transport kernel.
-// Required defines: K, IN (features padded to 64), WIDTH (hidden width, multiple of 64), DEPTH (hidden layers),
-// OUT (outputs padded to 64), DK (fit length, at most 8).
+// Required defines: K, IN (inputs padded to 64), WIDTH (hidden width, multiple of 64), DEPTH (hidden layers),
+// OUT (outputs padded to 64), DK (fit length, at most 8), NM (major species).
+// Tables: major[NM] (species indices), slot[K] (position of k in S or -1), wa/wb [K][NM+1] (Wilke a_kj, b_kj for
+// j = major[i], slot NM for b), dfit [K][NM+1][DK] (d_kj for j = major[i], slot NM for r(k)).
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
-#define NB 128
-#define HSTRIDE (WIDTH + 8)  // activation row stride (BF16 elements), padded against bank conflicts
+#define NB 64
 #define NF (K + 2)
+#define KW ((K + 3) / 4)  // species per wave in stage 1
+#define HS ((IN > WIDTH ? IN : WIDTH) + 8)  // activation row stride (BF16): an odd number of 16-byte units, so the
+                                             // 16 rows of a B fragment start in distinct bank groups
+static_assert(IN + 2 * (2 * NM + 1) <= HS, "stage-1 scratch does not fit the activation rows");
 
 __device__ __forceinline__ u16 to_bf16(float x) {
   unsigned u = __float_as_uint(x);
@@ -40,90 +52,188 @@ __device__ __forceinline__ float poly(const float* __restrict__ c, float l) {
 }
 
 struct Lds {
-  u16 h[NB][HSTRIDE];
-  float lnT[NB], lnP[NB], l[NB], rN[NB], lam[NB], mu[NB];
-  int inside[NB];
-};
+  u16 h[NB][HS];         // activations: state-major rows
+  float y[NF][NB];       // ln baselines, then the outputs
+  float part[4][3][NB];  // per-wave partial sums: Σn, then (Σ Xλ, Σ X/λ, Σ Xη)
+  int inside[4][NB];
+};  // 52 KB at K=53, WIDTH=256: three blocks per CU
 
-// One layer: z[o][s] = Σ_i W[o][i] h[s][i] for this wave's output rows (RT row tiles of 16 from row0).
+// Stage-1 scratch in the activation row of state st, past the IN feature columns: X_j of the major species at
+// [0, NM), 1/v_j at [NM, 2 NM), 1/v_b at 2 NM (floats).
+__device__ __forceinline__ void put(Lds& L, int st, int i, float x) { __builtin_memcpy(&L.h[st][IN + 2 * i], &x, 4); }
+__device__ __forceinline__ float get(const Lds& L, int st, int i) {
+  float x;
+  __builtin_memcpy(&x, &L.h[st][IN + 2 * i], 4);
+  return x;
+}
+
+// z[o][s] = Σ_i W[o][i] h[s][i] for the RT row tiles of 16 from row0 and the 4 column tiles of 16 states.
 template <int RT, int NIN>
-__device__ __forceinline__ void layer(const u16* __restrict__ W, int row0, Lds& L, f32x4 (&acc)[RT][NB / 16], int lane) {
+__device__ __forceinline__ void layer(const u16* __restrict__ W, int row0, Lds& L, f32x4 (&acc)[RT][4], int lane) {
+#pragma unroll
   for (int r = 0; r < RT; r++)
-    for (int c = 0; c < NB / 16; c++) acc[r][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  const int m = lane & 15, q = 4 * (lane >> 4);
-#pragma unroll 2
-  for (int k = 0; k < NIN; k += 16) {
-    bf16x4 a[RT];
-    for (int r = 0; r < RT; r++) a[r] = *(const bf16x4*)(W + (size_t)(row0 + 16 * r + m) * NIN + k + q);
-    for (int c = 0; c < NB / 16; c++) {
-      const bf16x4 b = *(const bf16x4*)&L.h[16 * c + m][k + q];
-      for (int r = 0; r < RT; r++) acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][c], 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[r][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int m = lane & 15, q = 8 * (lane >> 4);
+  const u16* wp = W + (size_t)(row0 + m) * NIN + q;
+  bf16x8 a[RT];
+#pragma unroll
+  for (int r = 0; r < RT; r++) a[r] = *(const bf16x8*)(wp + (size_t)16 * r * NIN);
+#pragma unroll
+  for (int k = 0; k < NIN; k += 32) {
+    bf16x8 an[RT];
+    if (k + 32 < NIN) {
+#pragma unroll
+      for (int r = 0; r < RT; r++) an[r] = *(const bf16x8*)(wp + (size_t)16 * r * NIN + k + 32);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const bf16x8 b = *(const bf16x8*)&L.h[16 * c + m][k + q];
+#pragma unroll
+      for (int r = 0; r < RT; r++) acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][c], 0, 0, 0);
+    }
+    if (k + 32 < NIN) {
+#pragma unroll
+      for (int r = 0; r < RT; r++) a[r] = an[r];
     }
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256, 3)  // 3 blocks (12 waves) per CU: 168 VGPRs, 52 KB LDS
 surrogate_transport(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n,
                     const u16* __restrict__ w16, const float* __restrict__ params, const long* __restrict__ offsets,
                     const float* __restrict__ f_mean, const float* __restrict__ f_rstd, const float* __restrict__ lo,
                     const float* __restrict__ hi, const float* __restrict__ o_mean, const float* __restrict__ o_std,
-                    const float* __restrict__ fit_c, const float* __restrict__ fit_v, const float* __restrict__ fit_db,
-                    const float* __restrict__ rcp_W, float T0, unsigned* __restrict__ fallback,
+                    const float* __restrict__ fit_c, const float* __restrict__ fit_v, const int* __restrict__ major,
+                    const int* __restrict__ slot, const float* __restrict__ wa, const float* __restrict__ wb,
+                    const float* __restrict__ dfit, const float* __restrict__ rcp_W, float T0, unsigned* __restrict__ fallback,
                     unsigned* __restrict__ fallback_count) {
   __shared__ Lds L;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const unsigned s0 = blockIdx.x * NB;
 
-  // 1. Per state (one thread each): mole fractions, baselines, features and the box test.
-  if (tid < NB) {
-    const int st = tid;
-    const unsigned s = s0 + st;
-    const bool valid = s < n;
-    int inside = valid;
-    const double Td = valid ? in[0][s] : 1000.0, Pd = valid ? in[1][s] : 101325.0;
-    const float T = (float)Td, lnP = (float)log(Pd), l = (float)log(Td / (double)T0);
-    float N = 0.f;
-    for (int k = 0; k < K; k++) N += (valid ? (float)in[2 + k][s] : 1.f / K) * rcp_W[k];
-    const float rN = 1.f / N, sqrtT = __builtin_sqrtf(T);
-    float a = 0.f, b = 0.f, m = 0.f;
-    for (int k = 0; k < K; k++) {
-      const float X = (valid ? (float)in[2 + k][s] : 1.f / K) * rcp_W[k] * rN;
+  // 1. One lane per state; wave w takes the species k = w + 4j.
+  const unsigned s = s0 + lane;
+  const bool valid = s < n;
+  const double Td = valid ? in[0][s] : 1000.0, Pd = valid ? in[1][s] : 101325.0;
+  const float T = (float)Td, lnP = (float)log(Pd), l = (float)log(Td / (double)T0), lnT = __builtin_logf(T);
+  const float sqrtT = __builtin_sqrtf(T);
+  float nk[KW];  // Y_k / W_k
+  float part = 0.f;
+#pragma unroll
+  for (int j = 0; j < KW; j++) {
+    const int k = wave + 4 * j;
+    nk[j] = 0.f;
+    if (k < K) {
+      nk[j] = (valid ? (float)in[2 + k][s] : 1.f / K) * rcp_W[k];
+      part += nk[j];
+    }
+  }
+  L.part[wave][0][lane] = part;
+  __syncthreads();
+  const float rN = 1.f / (L.part[0][0][lane] + L.part[1][0][lane] + L.part[2][0][lane] + L.part[3][0][lane]);
+  float a = 0.f, b = 0.f, mu = 0.f;
+  float xk[KW], vk[KW];
+  int inside = valid;
+#pragma unroll
+  for (int j = 0; j < KW; j++) {
+    const int k = wave + 4 * j;
+    xk[j] = 0.f;
+    vk[j] = 1.f;
+    if (k < K) {
+      const float X = nk[j] * rN;
       const float lam = poly(fit_c + k * DK, l) * sqrtT, v = poly(fit_v + k * DK, l);
+      xk[j] = X;
+      vk[j] = v;
       a += X * lam;
       b += X * __builtin_amdgcn_rcpf(lam);
-      m += X * v * v;
       const int f = 2 + k;
       if (!(X >= lo[f] && X <= hi[f])) inside = 0;
-      L.h[st][f] = to_bf16((X - f_mean[f]) * f_rstd[f]);
+      L.h[lane][f] = to_bf16((X - f_mean[f]) * f_rstd[f]);
+      const int i = slot[k];
+      if (i >= 0) {
+        put(L, lane, i, X);
+        put(L, lane, NM + i, __builtin_amdgcn_rcpf(v));
+      }
+      if (k == K - 1) put(L, lane, 2 * NM, __builtin_amdgcn_rcpf(v));
     }
-    if (!(T >= lo[0] && T <= hi[0]) || !(lnP >= lo[1] && lnP <= hi[1])) inside = 0;
-    L.h[st][0] = to_bf16((T - f_mean[0]) * f_rstd[0]);
-    L.h[st][1] = to_bf16((lnP - f_mean[1]) * f_rstd[1]);
-    for (int f = NF; f < IN; f++) L.h[st][f] = 0;
-    L.lnT[st] = __builtin_logf(T);
-    L.lnP[st] = lnP;
-    L.l[st] = l;
-    L.rN[st] = rN;
-    L.lam[st] = 0.5f * (a + __builtin_amdgcn_rcpf(b));
-    L.mu[st] = m * sqrtT;
-    L.inside[st] = inside;
+  }
+  __syncthreads();  // every wave has read part[*][0] and written its major species
+
+  // The pair sums over the major species (j outer: one LDS read of X_j and 1/v_j serves every own species).
+  float x_rest = 1.f;
+  for (int i = 0; i < NM; i++) x_rest -= get(L, lane, i);
+  x_rest = fmaxf(x_rest, 0.f);
+  float den[KW], S[KW];
+#pragma unroll
+  for (int j = 0; j < KW; j++) den[j] = S[j] = 0.f;
+  for (int i = 0; i < NM; i++) {
+    const float Xi = get(L, lane, i), rvi = get(L, lane, NM + i);
+    const int mi = major[i];
+#pragma unroll
+    for (int j = 0; j < KW; j++) {
+      const int k = wave + 4 * j;
+      if (k < K) {
+        const float t = wa[k * (NM + 1) + i] + wb[k * (NM + 1) + i] * (vk[j] * rvi);
+        den[j] += Xi * t * t;
+        if (mi != k) S[j] += Xi * __builtin_amdgcn_rcpf(poly(dfit + (k * (NM + 1) + i) * DK, l));
+      }
+    }
+  }
+  const float rv_b = get(L, lane, 2 * NM);
+#pragma unroll
+  for (int j = 0; j < KW; j++) {
+    const int k = wave + 4 * j;
+    if (k < K) {
+      const bool minor = slot[k] < 0;
+      const float X = xk[j], Xo = fmaxf(x_rest - (minor ? X : 0.f), 0.f);
+      const float t = wa[k * (NM + 1) + NM] + wb[k * (NM + 1) + NM] * (vk[j] * rv_b);
+      const float dn = den[j] + Xo * t * t + (minor ? X : 0.f);
+      const float Sk = S[j] + Xo * __builtin_amdgcn_rcpf(poly(dfit + (k * (NM + 1) + NM) * DK, l));
+      mu += X * vk[j] * vk[j] * __builtin_amdgcn_rcpf(dn);
+      const float Yk = nk[j] / rcp_W[k];
+      L.y[2 + k][lane] = __builtin_logf(fmaxf(1.f - Yk, 1e-30f)) - __builtin_logf(fmaxf(Sk, 1e-30f)) + 1.5f * lnT - lnP;
+    }
+  }
+  L.part[wave][0][lane] = a;
+  L.part[wave][1][lane] = b;
+  L.part[wave][2][lane] = mu;
+  L.inside[wave][lane] = inside;
+  __syncthreads();  // every wave has read the stage-1 scratch
+  for (int c = NF + wave; c < IN; c += 4) L.h[lane][c] = 0;
+  if (wave == 0) {
+    float A = 0.f, B = 0.f, Mu = 0.f;
+    int in_box = (T >= lo[0] && T <= hi[0]) && (lnP >= lo[1] && lnP <= hi[1]);
+    for (int w = 0; w < 4; w++) {
+      A += L.part[w][0][lane];
+      B += L.part[w][1][lane];
+      Mu += L.part[w][2][lane];
+      in_box &= L.inside[w][lane];
+    }
+    L.inside[0][lane] = in_box;
+    L.y[0][lane] = __builtin_logf(0.5f * (A + __builtin_amdgcn_rcpf(B)));
+    L.y[1][lane] = __builtin_logf(Mu * sqrtT);
+    L.h[lane][0] = to_bf16((T - f_mean[0]) * f_rstd[0]);
+    L.h[lane][1] = to_bf16((lnP - f_mean[1]) * f_rstd[1]);
   }
   __syncthreads();
 
-  // 2. Hidden layers, each wave WIDTH/4 output features for all NB states.
+  // 2. Hidden layers, each wave WIDTH/4 output rows for all NB states.
   constexpr int RH = WIDTH / 64;
   for (int d = 0; d < DEPTH; d++) {
     const long w_off = offsets[2 * d], b_off = offsets[2 * d + 1];
-    f32x4 acc[RH][NB / 16];
+    f32x4 acc[RH][4];
     if (d == 0)
       layer<RH, IN>(w16 + w_off, wave * (WIDTH / 4), L, acc, lane);
     else
       layer<RH, WIDTH>(w16 + w_off, wave * (WIDTH / 4), L, acc, lane);
     __syncthreads();  // every wave has read the previous activations
+#pragma unroll
     for (int r = 0; r < RH; r++) {
       const int o = wave * (WIDTH / 4) + 16 * r + 4 * (lane >> 4);
       const float b0 = params[b_off + o], b1 = params[b_off + o + 1], b2 = params[b_off + o + 2], b3 = params[b_off + o + 3];
-      for (int c = 0; c < NB / 16; c++) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
         const int st = 16 * c + (lane & 15);
         const f32x4 z = acc[r][c];
         u16 v[4] = {to_bf16(silu(z[0] + b0)), to_bf16(silu(z[1] + b1)), to_bf16(silu(z[2] + b2)), to_bf16(silu(z[3] + b3))};
@@ -133,36 +243,29 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
     __syncthreads();
   }
 
-  // 3. Output layer (OUT/4 features per wave), baseline and exponential; fallback list for states outside the box.
+  // 3. Output layer (OUT/4 rows per wave): y = exp(r σ + m + ln baseline), through LDS.
   constexpr int RO = OUT / 64;
-  f32x4 acc[RO][NB / 16];
-  layer<RO, WIDTH>(w16 + offsets[2 * DEPTH], wave * (OUT / 4), L, acc, lane);
-  const long b_off = offsets[2 * DEPTH + 1];
-  for (int r = 0; r < RO; r++)
-    for (int c = 0; c < NB / 16; c++) {
-      const int st = 16 * c + (lane & 15);
-      const unsigned s = s0 + st;
-      if (s >= n || !L.inside[st]) continue;
+  {
+    f32x4 acc[RO][4];
+    layer<RO, WIDTH>(w16 + offsets[2 * DEPTH], wave * (OUT / 4), L, acc, lane);
+    const long b_off = offsets[2 * DEPTH + 1];
+#pragma unroll
+    for (int r = 0; r < RO; r++)
+#pragma unroll
       for (int i = 0; i < 4; i++) {
         const int o = wave * (OUT / 4) + 16 * r + 4 * (lane >> 4) + i;
         if (o >= NF) continue;
-        float lnbase;
-        if (o == 0) {
-          lnbase = __builtin_logf(L.lam[st]);
-        } else if (o == 1) {
-          lnbase = __builtin_logf(L.mu[st]);
-        } else {
-          const int k = o - 2;
-          const float Yk = (float)in[2 + k][s], Xk = Yk * rcp_W[k] * L.rN[st];
-          lnbase = __builtin_logf(poly(fit_db + k * DK, L.l[st])) + 1.5f * L.lnT[st] - L.lnP[st] +
-                   __builtin_logf(fmaxf(1.f - Yk, 1e-30f)) - __builtin_logf(fmaxf(1.f - Xk, 1e-30f));
+        const float bo = params[b_off + o], so = o_std[o], mo = o_mean[o];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int st = 16 * c + (lane & 15);
+          L.y[o][st] = __builtin_expf((acc[r][c][i] + bo) * so + mo + L.y[o][st]);
         }
-        const float y = (acc[r][c][i] + params[b_off + o]) * o_std[o] + o_mean[o] + lnbase;
-        out[o][s] = (double)__builtin_expf(y);
       }
-    }
-  if (tid < NB) {
-    const unsigned s = s0 + tid;
-    if (s < n && !L.inside[tid]) fallback[atomicAdd(fallback_count, 1u)] = s;
   }
+  __syncthreads();
+  const int in_box = L.inside[0][lane];
+  if (valid && in_box)
+    for (int o = wave; o < NF; o += 4) out[o][s] = (double)L.y[o][lane];
+  if (wave == 0 && valid && !in_box) fallback[atomicAdd(fallback_count, 1u)] = s;
 }

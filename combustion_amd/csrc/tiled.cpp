@@ -470,10 +470,10 @@ std::vector<uint16_t> to_bf16(const std::vector<float>& v) {
   return out;
 }
 
-std::string surrogate_text(int K, const MlpLayout& lay, int DK) {
+std::string surrogate_text(int K, const MlpLayout& lay, int DK, int NM) {
   std::ostringstream s;
   s << "#define K " << K << "\n#define IN " << lay.in_pad << "\n#define WIDTH " << lay.width << "\n#define DEPTH " << lay.depth
-    << "\n#define OUT " << lay.out_pad << "\n#define DK " << DK << "\n"
+    << "\n#define OUT " << lay.out_pad << "\n#define DK " << DK << "\n#define NM " << NM << "\n"
     << surrogate_source;
   return s.str();
 }
@@ -489,6 +489,7 @@ class Surrogate : public SoAForm {
     const Mechanism& m = *p.mechanism;
     const int K = (int)m.species.len(), NF = K + 2;
     const MlpLayout lay(NF, w.width, w.depth, NF);
+    if (lay.width % 64) throw Unsupported("the surrogate kernel needs a hidden width that is a multiple of 64");
     if ((long)w.params.size() != lay.n_params) throw ArgumentError("surrogate parameters do not match the network size");
     for (const auto* v : {&w.f_mean, &w.f_rstd, &w.lo, &w.hi, &w.o_mean, &w.o_std})
       if ((int)v->size() != NF) throw ArgumentError("surrogate normalisation arrays need K+2 entries");
@@ -500,25 +501,45 @@ class Surrogate : public SoAForm {
     const Polynomials& poly = polynomials(p);
     const int D = poly.D, DK = D;
     if (D > 8) throw Unsupported("transport fits longer than 8 coefficients");
-    // Baselines (kernels/surrogate.hip): every species' conductivity and viscosity fits, d_kb against the bulk
-    // species b (species 0 for the bulk species itself), 1/W.
-    const int b = K - 1;
-    std::vector<float> fc, fv, fd, rw;
+    // Baselines (kernels/surrogate.hip): every species' conductivity and viscosity fits, 1/W, and per species k the
+    // Wilke coefficients and diffusion fits against the major species j = major[i] (slot NM: the bulk species b
+    // for Wilke, r(k) for diffusion).
+    const int b = K - 1, NM = (int)w.major.size();
+    if (NM < 1 || NM > K) throw ArgumentError("the number of major species is 1..K");
+    if (lay.in_pad + 2 * (2 * NM + 1) > std::max(lay.in_pad, (long)lay.width) + 8)
+      throw Unsupported("too many major species for the surrogate kernel's stage-1 scratch");
+    std::vector<int32_t> slot(K, -1);
+    int other = -1;
+    for (int i = 0; i < NM; i++) {
+      const int j = w.major[i];
+      if (j < 0 || j >= K || slot[j] >= 0) throw ArgumentError("major species must be distinct species indices");
+      slot[j] = i;
+      if (j != b && other < 0) other = j;
+    }
+    if (other < 0) throw ArgumentError("the major species need one besides the bulk species");
+    const auto& W = m.species.molar_mass;
+    std::vector<float> fc, fv, rw, wa, wb, df;
     for (int k = 0; k < K; k++) {
-      const int j = k == b ? 0 : b;
       for (int d = 0; d < D; d++) {
         fc.push_back((float)poly.conductivityIVT[k * D + d]);
         fv.push_back((float)poly.VviscosityIVVT[k * D + d]);
-        fd.push_back((float)poly.diffusivityITVT[((size_t)k * K + j) * D + d]);
       }
-      rw.push_back((float)(1. / m.species.molar_mass[k]));
+      rw.push_back((float)(1. / W[k]));
+      for (int i = 0; i <= NM; i++) {
+        const int j = i < NM ? w.major[i] : b;
+        const double a = 1. / std::pow(8. * (1. + W[k] / W[j]), 0.25);
+        wa.push_back((float)a);
+        wb.push_back((float)(a * std::pow(W[j] / W[k], 0.25)));
+        const int jd = i < NM ? w.major[i] : (k == b ? other : b);
+        for (int d = 0; d < D; d++) df.push_back((float)poly.diffusivityITVT[((size_t)k * K + jd) * D + d]);
+      }
     }
     std::vector<long> offsets;
     for (const auto& L : lay.layers) {
       offsets.push_back(L.w);
       offsets.push_back(L.b);
     }
-    kernel_.load(device::compile(surrogate_text(K, lay, DK), "surrogate_transport", hip_options(true)), "surrogate_transport");
+    kernel_.load(device::compile(surrogate_text(K, lay, DK, NM), "surrogate_transport", hip_options(true)), "surrogate_transport");
     d_w16_ = upload(to_bf16(w.params));
     d_params_ = upload(w.params);
     d_offsets_ = upload(offsets);
@@ -530,7 +551,11 @@ class Surrogate : public SoAForm {
     d_o_std_ = upload(w.o_std);
     d_bc_ = upload(fc);
     d_bv_ = upload(fv);
-    d_bd_ = upload(fd);
+    d_major_ = upload(w.major);
+    d_slot_ = upload(slot);
+    d_wa_ = upload(wa);
+    d_wb_ = upload(wb);
+    d_bd_ = upload(df);
     d_rw_ = upload(rw);
     T0_ = (float)p.chemistry->scales.T0;
     device::check(hipMalloc(&d_count_, sizeof(unsigned)), "hipMalloc");
@@ -538,7 +563,7 @@ class Surrogate : public SoAForm {
   ~Surrogate() override {
     for (void* ptr : {(void*)d_w16_, (void*)d_params_, (void*)d_offsets_, (void*)d_f_mean_, (void*)d_f_rstd_, (void*)d_lo_,
                       (void*)d_hi_, (void*)d_o_mean_, (void*)d_o_std_, (void*)d_bc_, (void*)d_bv_, (void*)d_bd_, (void*)d_rw_, (void*)d_count_,
-                      (void*)d_fallback_})
+                      (void*)d_fallback_, (void*)d_major_, (void*)d_slot_, (void*)d_wa_, (void*)d_wb_})
       if (ptr) (void)hipFree(ptr);
   }
   // States the last call sent to the plain transport kernel (waits for the stream).
@@ -562,8 +587,9 @@ class Surrogate : public SoAForm {
     device::check(hipMemsetAsync(d_count_, 0, sizeof(unsigned), stream), "hipMemsetAsync");
     unsigned n32 = (unsigned)n;
     void* args[] = {&in, &tout, &n32, &d_w16_, &d_params_, &d_offsets_, &d_f_mean_, &d_f_rstd_, &d_lo_, &d_hi_,
-                    &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_bd_, &d_rw_, &T0_, &d_fallback_, &d_count_};
-    const unsigned per_block = 128;
+                    &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_major_, &d_slot_, &d_wa_, &d_wb_, &d_bd_, &d_rw_, &T0_,
+                    &d_fallback_, &d_count_};
+    const unsigned per_block = 64;
     device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + per_block - 1) / per_block), 1, 1, 256, 1, 1, 0, stream, args,
                                         nullptr),
                   "launch surrogate_transport");
@@ -577,7 +603,8 @@ class Surrogate : public SoAForm {
   device::Kernel kernel_;
   uint16_t* d_w16_ = nullptr;
   float *d_params_ = nullptr, *d_f_mean_ = nullptr, *d_f_rstd_ = nullptr, *d_lo_ = nullptr, *d_hi_ = nullptr, *d_o_mean_ = nullptr,
-        *d_o_std_ = nullptr, *d_bc_ = nullptr, *d_bv_ = nullptr, *d_bd_ = nullptr, *d_rw_ = nullptr;
+        *d_o_std_ = nullptr, *d_bc_ = nullptr, *d_bv_ = nullptr, *d_bd_ = nullptr, *d_rw_ = nullptr, *d_wa_ = nullptr, *d_wb_ = nullptr;
+  int32_t *d_major_ = nullptr, *d_slot_ = nullptr;
   long* d_offsets_ = nullptr;
   float T0_ = 1000.f;
   unsigned *d_count_ = nullptr, *d_fallback_ = nullptr;

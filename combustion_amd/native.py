@@ -316,7 +316,8 @@ class Module:
 class SurrogateWeights(C.Structure):
     """cb_surrogate_weights."""
     _fields_ = [("width", C.c_int32), ("depth", C.c_int32)] + [
-        (name, C.POINTER(C.c_float)) for name in ("params", "feature_mean", "feature_rstd", "lower", "upper", "output_mean", "output_std")]
+        (name, C.POINTER(C.c_float)) for name in ("params", "feature_mean", "feature_rstd", "lower", "upper", "output_mean", "output_std")] + [
+        ("n_major", C.c_int32), ("major", C.POINTER(C.c_int32))]
 
 
 def _tensor(x, dtype):

@@ -3,14 +3,17 @@ MLP kernels (kernels/mlp.hip through cb_mlp_*), and the surrogate form (cb_hip_a
 
 The network maps the state's features f = (T, ln P, X_0..X_{K-1}) (mole fractions from the mass fractions Y) to
 r_o = ln(p_o / b_o) for the K+2 transport outputs p = [λ, μ, D_0..D_{K-1}] of the chemistry program
-(transport.rs:141-177), relative to baselines b that cost O(K) per state, from the transport fits in ℓ = ln(T/T0)
-(transport.rs:125-136; λ_k = c_k(ℓ)√T, η_k = v_k(ℓ)²√T, D_kb = d_kb(ℓ) T^1.5/P against the bulk species b):
-    λ_b = ½ (Σ X_k λ_k + 1/Σ X_k/λ_k)     the conductivity rule itself, which is O(K)
-    μ_b = Σ X_k η_k                         linear mixing in place of Wilke's O(K²) sum
-    D_b,k = D_kb (1 - Y_k)/(1 - X_k)        every D_kj replaced by D_kb (species 0 partners the bulk species)
-so the network learns what the O(K²) sums add. Features and targets are standardised with the training set's
-means and deviations; the box [min, max] of the training features is the domain where the network is used —
-states outside it are evaluated by the plain transport kernel.
+(transport.rs:141-177), relative to baselines b that cost O(K·M) per state instead of O(K²): the mixture rules
+with the pair sums over j restricted to M major species S (the M largest mole fractions seen in the training
+states) and the remaining mole fraction X_rest = 1 - Σ_S X_j given to the bulk species b, from the transport fits
+in ℓ = ln(T/T0) (transport.rs:125-136; λ_k = c_k(ℓ)√T, η_k = v_k(ℓ)²√T, D_kj = d_kj(ℓ) T^1.5/P):
+    λ_b = ½ (Σ X_k λ_k + 1/Σ X_k/λ_k)          the conductivity rule itself, which is O(K)
+    μ_b = Σ_k X_k η_k / den_k,  den_k = Σ_S X_j φ_kj + X'_k φ_kb + [k ∉ S] X_k      (Wilke, transport.rs:148-158)
+    D_b,k = (1 - Y_k) / (Σ_{S, j≠k} X_j / D_kj + X'_k / D_k,r(k))                    (transport.rs:162-177)
+with X'_k = X_rest - [k ∉ S] X_k and r(k) = b, or for k = b the first other major species; so the network learns
+what the minor species add. Features and targets are standardised with the training set's means and deviations;
+the box [min, max] of the training features is the domain where the network is used — states outside it are
+evaluated by the plain transport kernel.
 """
 from __future__ import annotations
 
@@ -23,22 +26,24 @@ from . import native
 
 
 class Trained:
-    """A trained surrogate: network parameters (flat FP32, cb_mlp layout) and normalisation."""
+    """A trained surrogate: network parameters (flat FP32, cb_mlp layout), normalisation and the major species of
+    its baselines."""
 
     FIELDS = ("params", "f_mean", "f_rstd", "lo", "hi", "o_mean", "o_std")
 
-    def __init__(self, width, depth, params, f_mean, f_rstd, lo, hi, o_mean, o_std):
+    def __init__(self, width, depth, params, f_mean, f_rstd, lo, hi, o_mean, o_std, major):
         self.width, self.depth = int(width), int(depth)
         self.params, self.f_mean, self.f_rstd = (np.ascontiguousarray(x, np.float32) for x in (params, f_mean, f_rstd))
         self.lo, self.hi, self.o_mean, self.o_std = (np.ascontiguousarray(x, np.float32) for x in (lo, hi, o_mean, o_std))
+        self.major = np.ascontiguousarray(major, np.int32)
 
     def save(self, path):
-        np.savez(path, width=self.width, depth=self.depth, **{k: getattr(self, k) for k in self.FIELDS})
+        np.savez(path, width=self.width, depth=self.depth, major=self.major, **{k: getattr(self, k) for k in self.FIELDS})
 
     @classmethod
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
-        return cls(int(z["width"]), int(z["depth"]), *(z[k] for k in cls.FIELDS))
+        return cls(int(z["width"]), int(z["depth"]), *(z[k] for k in cls.FIELDS), z["major"])
 
 
 def mole_fractions(mechanism, Y):
@@ -52,26 +57,56 @@ def features(mechanism, T, P, Y):
     return np.concatenate([np.asarray(T)[:, None], np.log(np.asarray(P))[:, None], X.T], axis=1)
 
 
-def log_baseline(mechanism, T, P, Y, D=4, T0=1000.0):
-    """ln b_o [n][K+2] (see the module docstring)."""
+def choose_major(mechanism, Y, m=16):
+    """The m species with the largest mole fraction over the states Y [K][n], largest first (int32)."""
+    return np.argsort(-mole_fractions(mechanism, Y).max(axis=1), kind="stable")[:m].astype(np.int32)
+
+
+def rest_partner(K, major):
+    """r(k): the bulk species K-1, or for the bulk itself the first other major species."""
+    r = np.full(K, K - 1, np.int32)
+    r[K - 1] = next(int(j) for j in major if j != K - 1)
+    return r
+
+
+def wilke_coefficients(W):
+    """(a, b) [K][K] with Wilke's φ_kj = (a_kj + b_kj v_k/v_j)², v = √η (transport.rs:148-158)."""
+    a = 1 / (8 * (1 + W[:, None] / W[None, :])) ** 0.25
+    return a, a * (W[None, :] / W[:, None]) ** 0.25
+
+
+def log_baseline(mechanism, T, P, Y, major, D=4, T0=1000.0):
+    """ln b_o [n][K+2] (see the module docstring) for the major species `major`."""
     c, v, d = mechanism.transport_fits(D, T0)
     K = mechanism.K
     b = K - 1
+    major = np.asarray(major, np.int64)
+    in_S = np.zeros(K, bool)
+    in_S[major] = True
+    r = rest_partner(K, major)
     T = np.asarray(T, np.float64)
     Y = np.asarray(Y, np.float64)
-    X = mole_fractions(mechanism, Y)
+    X = mole_fractions(mechanism, Y)                   # [K][n]
     l = np.log(T / T0)
     pw = np.stack([l ** i for i in range(D)], axis=1)  # [n][D]
-    lam = (pw @ c.T).T * np.sqrt(T)                      # [K][n]
-    eta = ((pw @ v.T).T) ** 2 * np.sqrt(T)
+    sqrtT = np.sqrt(T)
+    lam = (pw @ c.T).T * sqrtT                          # [K][n]
+    vk = (pw @ v.T).T                                   # [K][n]
+    a, bb = wilke_coefficients(mechanism.molar_mass)
+    X_rest = np.maximum(1 - X[major].sum(0), 0)
+    X_other = np.maximum(X_rest[None, :] - np.where(in_S[:, None], 0, X), 0)  # X'_k
+    den = np.zeros_like(X)
+    S = np.zeros_like(X)
+    for j in major:
+        den += X[j] * (a[:, j, None] + bb[:, j, None] * vk / vk[j]) ** 2
+        dkj = np.einsum("kd,nd->kn", d[:, j], pw)
+        S += np.where(np.arange(K)[:, None] == j, 0, X[j] / np.where(np.arange(K)[:, None] == j, 1, dkj))
+    den += X_other * (a[:, b, None] + bb[:, b, None] * vk / vk[b]) ** 2 + np.where(in_S[:, None], 0, X)
+    S += X_other / np.einsum("kd,nd->kn", d[np.arange(K), r], pw)
     out = np.empty((len(T), K + 2))
     out[:, 0] = np.log(0.5 * ((X * lam).sum(0) + 1 / (X / lam).sum(0)))
-    out[:, 1] = np.log((X * eta).sum(0))
-    partner = np.full(K, b)
-    partner[b] = 0
-    dk = d[np.arange(K), partner]  # [K][D]
-    Dkb = (pw @ dk.T) * (T ** 1.5 / np.asarray(P))[:, None]  # [n][K]
-    out[:, 2:] = np.log(Dkb) + np.log(np.maximum(1 - Y.T, 1e-30)) - np.log(np.maximum(1 - X.T, 1e-30))
+    out[:, 1] = np.log(sqrtT * (X * vk ** 2 / den).sum(0))
+    out[:, 2:] = (np.log(np.maximum(1 - Y, 1e-30)) - np.log(np.maximum(S, 1e-300)) + 1.5 * np.log(T) - np.log(np.asarray(P))).T
     return out
 
 
@@ -102,16 +137,26 @@ def initial_params(net, rng):
     return params
 
 
-def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384, lr=2e-3, seed=0, device="cuda", log=None):
-    """Fits the network to props [K+2][n] (the judge's outputs) on the GPU with the project's kernels."""
+def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384, lr=2e-3, seed=0, device="cuda", log=None,
+          hard=0.0, refresh=1000, common_scale=False, n_major=20):
+    """Fits the network to props [K+2][n] (the judge's outputs) on the GPU with the project's kernels.
+
+    hard > 0 mines hard examples: every `refresh` steps the network is evaluated on a random sample of the training
+    states and a fraction `hard` of each later batch is drawn from the 5% of that sample with the largest error
+    max_o |Δ ln p_o|. common_scale standardises every output by the largest deviation instead of its own, so that the
+    loss weighs each output's error in ln p_o alike (the relative error that the bound is stated in). The baselines
+    keep the pair terms of the n_major species with the largest mole fractions in the training states."""
     import torch
     K = mechanism.K
     NF = K + 2
     F = features(mechanism, T, P, Y)
-    R = np.log(np.asarray(props).T) - log_baseline(mechanism, T, P, Y)
+    major = choose_major(mechanism, Y, n_major)
+    R = np.log(np.asarray(props).T) - log_baseline(mechanism, T, P, Y, major)
     f_mean, f_std = F.mean(0), F.std(0)
     f_rstd = np.where(f_std > 0, 1 / np.maximum(f_std, 1e-300), 0.0)
     o_mean, o_std = R.mean(0), np.maximum(R.std(0), 1e-6)
+    if common_scale:
+        o_std = np.full_like(o_std, o_std.max())
     net = native.Network(NF, width, depth, NF)
     X = np.zeros((len(T), net.in_pad), np.float32)
     X[:, :NF] = (F - f_mean) * f_rstd
@@ -131,12 +176,20 @@ def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384
     perm = torch.randperm(n, generator=gen, device=dev)
     at = 0
     history = []
+    n_hard = int(round(hard * batch))
+    pool = None
+    scale = torch.from_numpy(o_std.astype(np.float32)).to(dev)
     for step in range(1, steps + 1):
-        if at + batch > n:
+        if n_hard and step % refresh == 0 and step < steps:
+            pool = _hard_pool(net, p, p16, Xd, Td, scale, NF, batch, ws, y, gen)
+        m = batch - n_hard if pool is not None else batch
+        if at + m > n:
             perm = torch.randperm(n, generator=gen, device=dev)
             at = 0
-        idx = perm[at:at + batch]
-        at += batch
+        idx = perm[at:at + m]
+        at += m
+        if pool is not None:
+            idx = torch.cat([idx, pool[torch.randint(len(pool), (n_hard,), generator=gen, device=dev)]])
         xb, tb = Xd[idx].contiguous(), Td[idx].contiguous()
         net.forward(p, p16, xb, y, ws)
         net.backward_mse(p16, y, tb, ws, grad)
@@ -147,9 +200,27 @@ def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384
             history.append((step, loss))
             if log:
                 log(f"step {step}: batch loss {loss:.3e}")
-    trained = Trained(width, depth, p.cpu().numpy(), f_mean, f_rstd, F.min(0), F.max(0), o_mean, o_std)
+    trained = Trained(width, depth, p.cpu().numpy(), f_mean, f_rstd, F.min(0), F.max(0), o_mean, o_std, major)
     trained.history = history
     return trained
+
+
+def _hard_pool(net, p, p16, Xd, Td, scale, NF, batch, ws, y, gen, sample=1 << 19, top=0.05):
+    """Indices of the training states with the largest max_o |Δ ln p_o| in a random sample."""
+    import torch
+    n = Xd.shape[0]
+    idx = torch.randperm(n, generator=gen, device=Xd.device)[:min(sample, n)]
+    errs = []
+    for i in range(0, len(idx), batch):
+        j = idx[i:i + batch]
+        xb = Xd[j].contiguous()
+        if len(j) < batch:
+            xb = torch.cat([xb, xb[:1].expand(batch - len(j), -1)])
+        net.forward(p, p16, xb, y, ws)
+        errs.append(((y[:len(j), :NF] - Td[j, :NF]).abs() * scale).amax(1))
+    e = torch.cat(errs)
+    k = max(1, int(top * len(e)))
+    return idx[torch.topk(e, k).indices]
 
 
 def assemble(program, trained):
@@ -160,7 +231,8 @@ def assemble(program, trained):
         if a.shape != (K + 2,):
             raise ValueError("normalisation arrays need K+2 entries")
     fp = C.POINTER(C.c_float)
-    w = native.SurrogateWeights(trained.width, trained.depth, *(a.ctypes.data_as(fp) for a in arrays))
+    w = native.SurrogateWeights(trained.width, trained.depth, *(a.ctypes.data_as(fp) for a in arrays), len(trained.major),
+                                trained.major.ctypes.data_as(C.POINTER(C.c_int32)))
     h = native._P()
     native._check(native.lib().cb_hip_assemble_surrogate(program._h, C.byref(w), C.byref(h)))
     mod = native.Module(h, program)
@@ -193,6 +265,9 @@ def main(argv=None):
     ap.add_argument("--depth", type=int, default=3)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--hard", type=float, default=0.0, help="fraction of each batch drawn from mined hard states")
+    ap.add_argument("--common-scale", action="store_true", help="one output scale for every output")
+    ap.add_argument("--major", type=int, default=20, help="major species kept in the baselines' pair sums")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     log = lambda m: print(f"[{time.strftime('%H:%M:%S')}] {m}", flush=True)
@@ -204,7 +279,8 @@ def main(argv=None):
     props = judge(mech, T, P, Y)
     log(f"judge transport in {time.time() - t:.1f} s")
     t = time.time()
-    trained = train(mech, T, P, Y, props, width=a.width, depth=a.depth, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    trained = train(mech, T, P, Y, props, width=a.width, depth=a.depth, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log,
+                    hard=a.hard, common_scale=a.common_scale, n_major=a.major)
     log(f"trained in {time.time() - t:.1f} s")
     trained.save(a.out)
     log(f"saved {a.out}")

@@ -114,21 +114,24 @@ void cb_module_free(cb_module* mod);
  * plus its BF16 copy, both device memory: per layer W [out][in] then b [out], with in/out padded to multiples
  * of 64 (cb_mlp_sizes). Activations are [n][in_pad] / [n][out_pad] FP32 row-major device arrays. */
 /* Surrogate transport form of a chemistry program (inputs [T, P, Y], CB_TRANSPORT with or without CB_RATES):
- * ω by the mixed-precision rates kernel, and λ, μ, D_k by the trained network (kernels/surrogate.hip), with the
- * FP32 transport kernel evaluating the states whose features fall outside the training box [lower, upper].
+ * ω by the FP32-core rates kernel (CB_FORM_FAST), and λ, μ, D_k by the trained network (kernels/surrogate.hip),
+ * with the FP32 transport kernel evaluating the states whose features fall outside the training box [lower, upper].
  * No counterpart in the reference. The network has n_in = n_out = K+2 (cb_mlp layout); features are
- * (T, ln P, Y_0..Y_{K-1}) standardised by (mean, 1/std); outputs are [λ, μ, D_0..D_{K-1}] as
- * ln(property / baseline) standardised by (mean, std), the baseline being the bulk species' conductivity and
- * viscosity fits and the binary diffusion fit against the bulk species (species 0 for the bulk itself). */
+ * (T, ln P, X_0..X_{K-1}) (mole fractions) standardised by (mean, 1/std); outputs are [λ, μ, D_0..D_{K-1}] as
+ * ln(property / baseline) standardised by (mean, std), the baselines costing O(K·n_major) per state: the
+ * mixture rules with their pair sums over j restricted to the major species and the rest of the mole fraction
+ * given to the bulk species (the last species; for the bulk itself, the first other major species). */
 typedef struct cb_surrogate_weights {
   int32_t width, depth;
   const float* params;          /* cb_mlp_sizes(K+2, width, depth, K+2) n_params values */
   const float* feature_mean;    /* [K+2] */
   const float* feature_rstd;    /* [K+2] */
-  const float* lower;           /* [K+2] training box, in feature units (T, ln P, Y) */
+  const float* lower;           /* [K+2] training box, in feature units (T, ln P, X) */
   const float* upper;           /* [K+2] */
   const float* output_mean;     /* [K+2] */
   const float* output_std;      /* [K+2] */
+  int32_t n_major;              /* 1..K, at least one major species other than the bulk */
+  const int32_t* major;         /* [n_major] distinct species indices */
 } cb_surrogate_weights;
 int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w, cb_module** out);
 /* States of the last cb_hip_call on a surrogate module that went to the plain transport kernel (waits for the
