@@ -8,8 +8,9 @@ FP64 oracle on a sample of the same states, and — when a trained network is pr
 2^20 held-out reactor-trajectory states, FP64 against FP32-core rates + network transport. Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--states N] [--no-cpu] [--surrogate PATH]
-  python bench.py --only rates|transport|fp64|mixed|fast|copy [--steps K]   (one launch kind alone, for rocprofv3 runs;
-      copy is a device-to-device copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
+  python bench.py --only rates|transport|fp64|mixed|fast|surrogate|copy [--steps K]   (one launch kind alone, for
+      rocprofv3 runs; surrogate runs the surrogate form on 2^20 held-out trajectory states; copy is a device-to-device
+      copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
 """
 import argparse
 import json
@@ -128,7 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--surrogate", default=SURROGATE)
     ap.add_argument("--error-sample", type=int, default=8192)
-    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "fast", "copy"])
+    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "fast", "surrogate", "copy"])
     ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -150,6 +151,14 @@ def main():
     inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
     prog = M.chemistry()
     outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
+    if args.only == "surrogate":
+        from combustion_amd import reactor
+        T, P, Y = reactor.trajectories(M, 2048, N, seed=4242, max_steps=800, batch=2048)
+        inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
+        mod = sur.assemble(prog, sur.Trained.load(args.surrogate))
+        _, ms = time_module(mod, inputs, outputs, args.steps, 1)
+        print(json.dumps({"only": "surrogate", "states": N, "steps": args.steps, "kernel_ms": ms, "fallback": sur.fallback_count(mod)}))
+        return
     if args.only:
         return only(args, M, inputs, N)
 

@@ -109,7 +109,8 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
                     const float* __restrict__ dfit, const float* __restrict__ rcp_W, float T0, unsigned* __restrict__ fallback,
                     unsigned* __restrict__ fallback_count) {
   __shared__ Lds L;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // The wave index in an SGPR: every table indexed by a species k = wave + 4j is then read with scalar loads.
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const unsigned s0 = blockIdx.x * NB;
 
   // 1. One lane per state; wave w takes the species k = w + 4j.
@@ -160,39 +161,41 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
   }
   __syncthreads();  // every wave has read part[*][0] and written its major species
 
-  // The pair sums over the major species (j outer: one LDS read of X_j and 1/v_j serves every own species).
+  // The pair sums over the major species: their X_j and 1/v_j in registers, then per own species k (wave-uniform)
+  // one pass over its table rows (contiguous, so they arrive in a few wide scalar loads).
+  float Xm[NM], rvm[NM];
   float x_rest = 1.f;
-  for (int i = 0; i < NM; i++) x_rest -= get(L, lane, i);
-  x_rest = fmaxf(x_rest, 0.f);
-  float den[KW], S[KW];
 #pragma unroll
-  for (int j = 0; j < KW; j++) den[j] = S[j] = 0.f;
   for (int i = 0; i < NM; i++) {
-    const float Xi = get(L, lane, i), rvi = get(L, lane, NM + i);
-    const int mi = major[i];
-#pragma unroll
-    for (int j = 0; j < KW; j++) {
-      const int k = wave + 4 * j;
-      if (k < K) {
-        const float t = wa[k * (NM + 1) + i] + wb[k * (NM + 1) + i] * (vk[j] * rvi);
-        den[j] += Xi * t * t;
-        if (mi != k) S[j] += Xi * __builtin_amdgcn_rcpf(poly(dfit + (k * (NM + 1) + i) * DK, l));
-      }
-    }
+    Xm[i] = get(L, lane, i);
+    rvm[i] = get(L, lane, NM + i);
+    x_rest -= Xm[i];
   }
+  x_rest = fmaxf(x_rest, 0.f);
   const float rv_b = get(L, lane, 2 * NM);
 #pragma unroll
   for (int j = 0; j < KW; j++) {
     const int k = wave + 4 * j;
     if (k < K) {
-      const bool minor = slot[k] < 0;
+      const float* __restrict__ ak = wa + k * (NM + 1);
+      const float* __restrict__ bk = wb + k * (NM + 1);
+      const float* __restrict__ dk = dfit + k * (NM + 1) * DK;
+      const int own = slot[k];
+      float den = 0.f, S = 0.f;
+#pragma unroll
+      for (int i = 0; i < NM; i++) {
+        const float t = ak[i] + bk[i] * (vk[j] * rvm[i]);
+        den += Xm[i] * t * t;
+        if (i != own) S += Xm[i] * __builtin_amdgcn_rcpf(poly(dk + i * DK, l));
+      }
+      const bool minor = own < 0;
       const float X = xk[j], Xo = fmaxf(x_rest - (minor ? X : 0.f), 0.f);
-      const float t = wa[k * (NM + 1) + NM] + wb[k * (NM + 1) + NM] * (vk[j] * rv_b);
-      const float dn = den[j] + Xo * t * t + (minor ? X : 0.f);
-      const float Sk = S[j] + Xo * __builtin_amdgcn_rcpf(poly(dfit + (k * (NM + 1) + NM) * DK, l));
-      mu += X * vk[j] * vk[j] * __builtin_amdgcn_rcpf(dn);
+      const float t = ak[NM] + bk[NM] * (vk[j] * rv_b);
+      den += Xo * t * t + (minor ? X : 0.f);
+      S += Xo * __builtin_amdgcn_rcpf(poly(dk + NM * DK, l));
+      mu += X * vk[j] * vk[j] * __builtin_amdgcn_rcpf(den);
       const float Yk = nk[j] / rcp_W[k];
-      L.y[2 + k][lane] = __builtin_logf(fmaxf(1.f - Yk, 1e-30f)) - __builtin_logf(fmaxf(Sk, 1e-30f)) + 1.5f * lnT - lnP;
+      L.y[2 + k][lane] = __builtin_logf(fmaxf(1.f - Yk, 1e-30f)) - __builtin_logf(fmaxf(S, 1e-30f)) + 1.5f * lnT - lnP;
     }
   }
   L.part[wave][0][lane] = a;

@@ -65,9 +65,10 @@ def test_inference_kernel_matches_oracle_network(gpu, small):
     got = evaluate(mod, T, P, Y, gpu)
     print("fallback", surrogate.fallback_count(mod), "of", n, flush=True)
     F = surrogate.features(M, T, P, Y)
-    margin = 1e-5 * (trained.hi - trained.lo)  # states at the box's edge may fall back after FP32 rounding
+    # The kernel tests its FP32 features against the FP32 box: compare only the states clear of the box's edges
+    # by more than FP32 rounding (1e-5 of the bound; a bound of 0 is exact).
     lo, hi = trained.lo.astype(np.float64), trained.hi.astype(np.float64)
-    inside = np.all(((F >= lo + margin) | (F == lo)) & ((F <= hi - margin) | (F == hi)), axis=1)
+    inside = np.all((F >= lo + 1e-5 * np.abs(lo)) & (F <= hi - 1e-5 * np.abs(hi)), axis=1)
     assert inside.sum() > 0.9 * n
     net = native.Network(NF, trained.width, trained.depth, NF)
     x = np.zeros((n, net.in_pad))
@@ -95,24 +96,23 @@ def held_out(gpu):
     fb = surrogate.fallback_count(mod)
     print(f"held-out: max {err.max():.2e}, p99 {np.percentile(err, 99):.2e}, median {np.median(err):.2e}, "
           f"λ {err[0].max():.2e}, μ {err[1].max():.2e}, D {err[2:].max():.2e}, fallback {fb}/{len(T)}; "
-          f"baselines alone: max {base.max():.2e}, median {np.median(base):.2e}", flush=True)
+          f"baselines alone: max {base.max():.2e}, p99 {np.percentile(base, 99):.2e}, median {np.median(base):.2e}",
+          flush=True)
     return err, base
 
 
 @pytest.mark.gpu
 def test_trained_network_gain_on_held_out_trajectories(gpu):
-    """What the network gains over the method without it (the O(K) baselines), on held-out trajectory states:
-    at least 50x lower median and max relative error, and 99 % of the outputs within 2e-3 of the FP64 oracle."""
+    """What the network gains over the method without it (the O(K·M) baselines alone), on held-out trajectory
+    states: at least 5x lower max and 2x lower 99th-percentile relative error. (The median rises: where the
+    baselines are exact to 1e-11 the network's FP32/BF16 output adds ~1e-7.)"""
     err, base = held_out(gpu)
-    assert np.median(err) * 50 <= np.median(base)
-    assert err.max() * 50 <= base.max()
-    assert np.percentile(err, 99) <= 2e-3
+    assert err.max() * 5 <= base.max()
+    assert np.percentile(err, 99) * 2 <= np.percentile(base, 99)
 
 
 @pytest.mark.gpu
-@pytest.mark.xfail(strict=True, reason="not met: the committed network's max error on held-out trajectory states is "
-                                       "about 1e-2 (99th percentile about 1.5e-3), at states near ignition")
 def test_trained_network_meets_the_stated_bound(gpu):
-    """The stated bound: every held-out output within 1e-3 of the FP64 oracle."""
+    """The stated bound: every held-out output within 1e-3 of the FP64 oracle (fallback states included)."""
     err, _ = held_out(gpu)
     assert err.max() <= SURROGATE_TOL
