@@ -57,6 +57,9 @@ TransportScales transport_scales(const Mechanism& m, double T0 = 1000.);
 // 1/V = (P/R)/T / Σ_j Y_j/W_j, so C_k = max(0, n_k)/V for k < K-1 and the bulk species closes C_tot = P/(R T).
 struct ChemistryOptions {
   bool rates = true, transport = true, rhs = false;
+  // Register-lean viscosity for the one-lane-per-state kernels: v_k is evaluated again (in Horner form) where
+  // Wilke's k-th row needs it instead of living beside 1/v_j through the whole sum; the same values to rounding.
+  bool lean_viscosity = false;
 };
 ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
 

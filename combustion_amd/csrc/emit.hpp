@@ -8,6 +8,7 @@ namespace combustion {
 
 struct StraightOptions {
   unsigned block_size = 256;
+  unsigned min_waves = 0;    // > 0: waves per SIMD the register allocation must leave room for (__launch_bounds__)
   bool reciprocal = false;   // a / b as a * (1/b), 1/b by v_rcp_f64 and two Newton steps (within 1 ulp of 1/b)
   bool f32 = false;          // evaluate in FP32 (inputs and outputs stay FP64 arrays); exp/log/rcp by the hardware ops
   // FP64 arithmetic with FP32 cores for exp and ln: e^x = 2^n · 2^f with the FP64 reduction x log2 e = n + f (|f| ≤ ½)

@@ -46,7 +46,9 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
     s << "__device__ __forceinline__ float rcp32(float b) { return __builtin_amdgcn_rcpf(b); }\n"
          "__device__ __forceinline__ float exp32(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }\n"
          "__device__ __forceinline__ float log32(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }\n";
-  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block_size << ") ";
+  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block_size;
+  if (o.min_waves) s << ", " << o.min_waves;
+  s << ") ";
   s << name
     << "(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n, unsigned offset) {\n"
     << "  const unsigned id = offset + blockIdx.x * blockDim.x + threadIdx.x;\n"

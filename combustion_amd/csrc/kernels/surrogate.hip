@@ -27,6 +27,7 @@
 // j = major[i], slot NM for b), dfit [K][NM+1][DK] (d_kj for j = major[i], slot NM for r(k)).
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
@@ -239,8 +240,9 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
       for (int c = 0; c < 4; c++) {
         const int st = 16 * c + (lane & 15);
         const f32x4 z = acc[r][c];
-        u16 v[4] = {to_bf16(silu(z[0] + b0)), to_bf16(silu(z[1] + b1)), to_bf16(silu(z[2] + b2)), to_bf16(silu(z[3] + b3))};
-        __builtin_memcpy(&L.h[st][o], v, 8);
+        // (__bf16) rounds to nearest even like to_bf16, two values per v_cvt_pk_bf16_f32
+        const bf16x4 v = {(__bf16)silu(z[0] + b0), (__bf16)silu(z[1] + b1), (__bf16)silu(z[2] + b2), (__bf16)silu(z[3] + b3)};
+        __builtin_memcpy(&L.h[st][o], &v, 8);
       }
     }
     __syncthreads();

@@ -279,12 +279,25 @@ E conductivityNIVT(Block& f, const std::vector<double>& P, int D, E N, const std
   return f.add(f.div(*A, N), f.div(N, *B));
 }
 
+// Σ_m P[m] lnT^m in Horner form (lnT[1] = ln T): a node distinct from dot_consts' power-basis sum.
+E horner(Block& f, const double* P, int D, const std::vector<E>& lnT) {
+  E e = f.c(P[D - 1]);
+  for (int m = D - 2; m >= 0; m--) e = f.add(f.mul(e, lnT[1]), f.c(P[m]));
+  return e;
+}
+
 // transport.rs:148-158
-E viscosityIVT(Block& f, const std::vector<double>& W, const std::vector<double>& P, int D, const std::vector<E>& lnT, const std::vector<E>& x) {
+E viscosityIVT(Block& f, const std::vector<double>& W, const std::vector<double>& P, int D, const std::vector<E>& lnT, const std::vector<E>& x,
+               bool lean) {
   const size_t K = x.size();
   std::vector<E> v, r;
-  for (size_t k = 0; k < K; k++) v.push_back(f.defv(dot_consts(f, &P[k * D], lnT), "VviscosityIVVT" + std::to_string(k)));
-  for (size_t k = 0; k < K; k++) r.push_back(f.defv(f.div(f.c(1.), v[k]), "rcp_VviscosityIVVT" + std::to_string(k)));
+  if (lean && D >= 2) {
+    for (size_t k = 0; k < K; k++) r.push_back(f.defv(f.div(f.c(1.), dot_consts(f, &P[k * D], lnT)), "rcp_VviscosityIVVT" + std::to_string(k)));
+    for (size_t k = 0; k < K; k++) v.push_back(horner(f, &P[k * D], D, lnT));
+  } else {
+    for (size_t k = 0; k < K; k++) v.push_back(f.defv(dot_consts(f, &P[k * D], lnT), "VviscosityIVVT" + std::to_string(k)));
+    for (size_t k = 0; k < K; k++) r.push_back(f.defv(f.div(f.c(1.), v[k]), "rcp_VviscosityIVVT" + std::to_string(k)));
+  }
   std::vector<E> outer;
   for (size_t k = 0; k < K; k++) {
     std::vector<E> inner;
@@ -331,7 +344,7 @@ Scaled scale(const Polynomials& p, double T0, double viscosity, double conductiv
 
 // transport.rs:185-209 from the mixture inputs already in scope; returns [λ', μ', ρD'_k...] before the caller's scaling.
 std::vector<E> properties_body(Block& f, const Species& species, const Polynomials& p, const Scaled& sc, E N, E theta,
-                               const std::vector<E>& nonbulk, E& mean_molar_massN) {
+                               const std::vector<E>& nonbulk, E& mean_molar_massN, bool lean = false) {
   const size_t K = species.len();
   const int D = p.D;
   E lnT = f.defv(f.ln(theta), "lnT");
@@ -350,7 +363,7 @@ std::vector<E> properties_body(Block& f, const Species& species, const Polynomia
   E mean_molar_mass_VTN = f.defv(f.mul(mean_molar_massN, VT), "mean_molar_mass_VTN");
   std::vector<E> out;
   out.push_back(f.mul(conductivityNIVT(f, sc.conductivity, D, N, powers, X), VT));
-  out.push_back(f.mul(VT, viscosityIVT(f, species.molar_mass, sc.viscosity, D, powers, X)));
+  out.push_back(f.mul(VT, viscosityIVT(f, species.molar_mass, sc.viscosity, D, powers, X, lean)));
   auto d = density_diffusivity(f, species.molar_mass, sc.diffusivity, D, mean_molar_mass_VTN, VT, powers, X);
   out.insert(out.end(), d.begin(), d.end());
   return out;
@@ -371,9 +384,9 @@ ast::Function properties(const Species& species, const Polynomials& p, double T0
 }
 
 // Transport section of the per-state program: physical λ, μ and D_k = ρD_k/ρ with ρ = Σ_k W_k C_k.
-std::vector<E> transport_outputs(Block& f, const Species& s, const Polynomials& p, const TransportScales& scales, E N, E theta,
+std::vector<E> transport_outputs(Block& f, const Species& s, const Polynomials& p, const TransportScales& scales, E N, E theta, bool lean,
                                  const std::vector<E>& x, E& mean_molar_massN) {
-  auto out = properties_body(f, s, p, scale(p, scales.T0, scales.viscosity, scales.conductivity), N, theta, x, mean_molar_massN);
+  auto out = properties_body(f, s, p, scale(p, scales.T0, scales.viscosity, scales.conductivity), N, theta, x, mean_molar_massN, lean);
   out[0] = f.mul(f.c(scales.conductivity), out[0]);
   out[1] = f.mul(f.c(scales.viscosity), out[1]);
   E scale_D = f.defv(f.div(f.c(scales.viscosity), mean_molar_massN), "η0/ρ");
