@@ -40,6 +40,15 @@ double omega22(const Species& s, size_t a, size_t b, double T);
 double omega11(const Species& s, size_t a, size_t b, double T);
 }  // namespace transport
 
+// Device-function export (cuda.rs:140-350): the per-state functions of the reference's nekRK export, with its
+// input names. Rates: inputs [pressure_R, lnT, T, T2, T3, T4, rcpT, concentrations_0..K-1], outputs ω_0..ω_{Ka-1}.
+ast::Function rates_device_function(const Mechanism& m);
+// Transport (which = 1 conductivityNIVT, 2 viscosityIVT, 3 density_diffusivity): inputs [sum_mole_proportions]
+// (1) or [mean_molar_mass_VTN, VT] (3), then lnT, lnT2.. (powers 1..D-1 of ln(T/T0)) and mole_proportions_0..K-1;
+// one output (1, 2) or K (3).
+ast::Function transport_device_function(const Species& species, const Polynomials& p, double T0, double viscosity, double conductivity,
+                                        int which);
+
 // transport.rs:179-214: inputs [Σx, T/T0, x_0..x_{K-2}], outputs [λ/λ0, μ/η0, ρD_k/η0].
 ast::Function properties(const Species& species, const Polynomials& p, double T0, double viscosity, double conductivity);
 

@@ -350,6 +350,33 @@ int cb_program_from_linear(int32_t n_inputs, int64_t n_instructions, const uint8
   });
 }
 
+int cb_export_device_function(const cb_mechanism* m, int32_t which, int32_t D, const char* prefix, char* buffer, size_t capacity,
+                               size_t* needed) {
+  std::string text;
+  int status = guard([&] {
+    require(m, "null mechanism");
+    require(which >= 0 && which <= 3, "which is CB_EXPORT_RATES, _CONDUCTIVITY, _VISCOSITY or _DIFFUSIVITY");
+    require(D >= 2 && D <= 8, "D out of range");
+    const std::string pre = prefix ? prefix : "";
+    if (which == 0) {
+      text = emit_device_function(ast::linearize(rates_device_function(*m->m)), {"pressure_R", "lnT", "T", "T2", "T3", "T4", "rcpT"},
+                                  "concentrations", pre + "species_rates", DeviceOutput::Out);
+      return;
+    }
+    const TransportScales sc = transport_scales(*m->m);
+    const Polynomials p = fit_polynomials(m->m->species, sc.T0, D);
+    std::vector<std::string> scalars;
+    if (which == 1) scalars.push_back("sum_mole_proportions");
+    if (which == 3) scalars = {"mean_molar_mass_VTN", "VT"};
+    for (int k = 1; k < D; k++) scalars.push_back(k == 1 ? "lnT" : "lnT" + std::to_string(k));
+    const char* names[] = {"", "conductivityNIVT", "viscosityIVT", "density_diffusivity"};
+    const ast::Linear l = ast::linearize(transport_device_function(m->m->species, p, sc.T0, sc.viscosity, sc.conductivity, which));
+    text = emit_device_function(l, scalars, "mole_proportions", pre + names[which], which == 3 ? DeviceOutput::Strided : DeviceOutput::Return);
+  });
+  if (status) return status;
+  return write_text(text, buffer, capacity, needed);
+}
+
 int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed) {
   std::string text;
   int status = guard([&] {

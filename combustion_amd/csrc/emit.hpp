@@ -19,6 +19,14 @@ struct StraightOptions {
 // Straight-line kernel: signature (const double* const* in, double* const* out, unsigned n, unsigned offset).
 std::string emit_straight(const ast::Linear& l, const std::string& name, const StraightOptions& o);
 
+// A per-state device function in the form of the reference's nekRK export (cuda.rs:296-318): `__NEKRK_DEVICE__`
+// prefix, `dfloat` arithmetic, the first n_scalar inputs as named scalar parameters and the rest as the array
+// parameter `array[]`. Outputs: DeviceReturn returns the single output; DeviceOut writes out[k];
+// DeviceStrided writes out[k*n_states+id] (parameters `unsigned int id` first and `unsigned int n_states` last).
+enum class DeviceOutput { Return, Out, Strided };
+std::string emit_device_function(const ast::Linear& l, const std::vector<std::string>& scalar_names, const std::string& array,
+                                 const std::string& name, DeviceOutput mode);
+
 // Mixed-precision rates kernel template (kernels/rates_mixed.hip), embedded at build time.
 extern const char* const rates_mixed_source;
 // Mixture-averaged transport kernel template (kernels/transport_tiled.hip).

@@ -100,4 +100,74 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   return s.str();
 }
 
+std::string emit_device_function(const ast::Linear& l, const std::vector<std::string>& scalar_names, const std::string& array,
+                                 const std::string& name, DeviceOutput mode) {
+  using ast::LOp;
+  const int n_scalar = (int)scalar_names.size();
+  if (n_scalar > l.n_input) throw ast::Error("device function: more scalar names than inputs");
+  if (mode == DeviceOutput::Return && l.output.size() != 1) throw ast::Error("device function: Return needs one output");
+  auto reg = [&](int32_t r) -> std::string {
+    if (r < 0) return "";  // an operand the instruction does not have
+    if (r < n_scalar) return scalar_names[r];
+    if (r < l.n_input) return array + "[" + std::to_string(r - n_scalar) + "]";
+    return "r" + std::to_string(r);
+  };
+  auto num = [](double x) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.17g", x);
+    std::string s = buf;
+    if (s.find_first_of(".eEn") == std::string::npos) s += ".";  // a floating literal (inf/nan never occur)
+    return s;
+  };
+  std::ostringstream s;
+  s << "__NEKRK_DEVICE__ " << (mode == DeviceOutput::Return ? "dfloat " : "void ") << name << "(";
+  std::vector<std::string> params;
+  if (mode == DeviceOutput::Strided) params.push_back("unsigned int id");
+  for (const std::string& n : scalar_names) params.push_back("dfloat " + n);
+  if (l.n_input > n_scalar) params.push_back("const dfloat " + array + "[]");
+  if (mode != DeviceOutput::Return) params.push_back("dfloat out[]");
+  if (mode == DeviceOutput::Strided) params.push_back("unsigned int n_states");
+  for (size_t i = 0; i < params.size(); i++) s << (i ? ", " : "") << params[i];
+  s << ") {\n";
+  std::vector<std::vector<size_t>> stores(l.n_registers);
+  for (size_t o = 0; o < l.output.size(); o++)
+    if (l.output[o] >= l.n_input) stores[l.output[o]].push_back(o);
+  auto store = [&](size_t o, const std::string& v) {
+    if (mode == DeviceOutput::Return) s << "  return " << v << ";\n";
+    else if (mode == DeviceOutput::Out) s << "  out[" << o << "] = " << v << ";\n";
+    else s << "  out[" << o << "*n_states+id] = " << v << ";\n";
+  };
+  for (const ast::Instruction& i : l.code) {
+    const std::string d = reg(i.dst), a = reg(i.a), b = reg(i.b), c = reg(i.c);
+    if (i.op == LOp::LessOrEqual) {
+      s << "  const bool " << d << " = " << a << " <= " << b << ";\n";
+      continue;
+    }
+    s << "  const dfloat " << d << " = ";
+    switch (i.op) {
+      case LOp::Const: s << num(i.k); break;
+      case LOp::Neg: s << "-" << a; break;
+      case LOp::Min: s << "fmin(" << a << ", " << b << ")"; break;
+      case LOp::Max: s << "fmax(" << a << ", " << b << ")"; break;
+      case LOp::Add: s << a << " + " << b; break;
+      case LOp::Sub: s << a << " - " << b; break;
+      case LOp::Mul: s << a << " * " << b; break;
+      case LOp::Div: s << a << " / " << b; break;
+      case LOp::Sqrt: s << "sqrt(" << a << ")"; break;
+      case LOp::Exp: s << "exp(" << a << ")"; break;
+      case LOp::Ln: s << "log(" << a << ")"; break;
+      case LOp::Sq: s << a << " * " << a; break;
+      case LOp::Select: s << a << " ? " << b << " : " << c; break;
+      default: break;
+    }
+    s << ";\n";
+    if (mode != DeviceOutput::Return)
+      for (size_t o : stores[i.dst]) store(o, d);
+  }
+  for (size_t o = 0; o < l.output.size(); o++)
+    if (l.output[o] < l.n_input || mode == DeviceOutput::Return) store(o, reg(l.output[o]));
+  s << "}\n";
+  return s.str();
+}
+
 }  // namespace combustion

@@ -374,6 +374,19 @@ ast::Function rates(const Mechanism& m) {
   return f.finish(out);
 }
 
+// The rates function of the device-function export (cuda.rs:210-226): ω from the temperature powers and all K
+// concentrations.
+ast::Function rates_device_function(const Mechanism& m) {
+  const size_t K = m.species.len();
+  Block f(names({"pressure_R", "lnT", "T", "T2", "T3", "T4", "rcpT"}, K, "concentrations"));
+  Temperatures T{f.v(1), f.v(2), f.v(3), f.v(4), f.v(5), f.v(6), -1};
+  T.rcpT2 = f.defv(f.mul(T.rcpT, T.rcpT), "rcpT2");
+  E concentration = f.defv(f.mul(f.v(0), T.rcpT), "concentration");
+  std::vector<E> C;
+  for (size_t k = 0; k < K; k++) C.push_back(f.v(7 + (int32_t)k));
+  return f.finish(species_rates(f, m, T, concentration, C));
+}
+
 ast::Function reactions_rates(const Mechanism& m) {
   const size_t K = m.species.len();
   Block f(names({"pressure_", "T", "volume"}, K - 1, "active_amounts"));

@@ -383,6 +383,40 @@ ast::Function properties(const Species& species, const Polynomials& p, double T0
   return f.finish(out);
 }
 
+// The transport functions of the device-function export (cuda.rs:150-208): inputs as the reference names them,
+// lnT^m for m = 1..D-1 (ln of T/T0) and all K mole proportions, coefficients scaled as in transport.rs:180-182.
+ast::Function transport_device_function(const Species& species, const Polynomials& p, double T0, double viscosity, double conductivity,
+                                        int which) {
+  const size_t K = species.len();
+  const int D = p.D;
+  std::vector<std::string> names;
+  if (which == 1) names.push_back("sum_mole_proportions");
+  if (which == 3) {
+    names.push_back("mean_molar_mass_VTN");
+    names.push_back("VT");
+  }
+  const size_t first_power = names.size();
+  for (int m = 1; m < D; m++) names.push_back(m == 1 ? "lnT" : "lnT" + std::to_string(m));
+  const size_t first_x = names.size();
+  for (size_t k = 0; k < K; k++) names.push_back("mole_proportions" + std::to_string(k));
+  Block f(names);
+  std::vector<E> powers = {f.c(1.)};
+  for (int m = 1; m < D; m++) powers.push_back(f.v((int32_t)(first_power + m - 1)));
+  std::vector<E> x;
+  for (size_t k = 0; k < K; k++) x.push_back(f.v((int32_t)(first_x + k)));
+  const Scaled sc = scale(p, T0, viscosity, conductivity);
+  std::vector<E> out;
+  if (which == 1)
+    out.push_back(conductivityNIVT(f, sc.conductivity, D, f.v(0), powers, x));
+  else if (which == 2)
+    out.push_back(viscosityIVT(f, species.molar_mass, sc.viscosity, D, powers, x, false));
+  else if (which == 3)
+    out = density_diffusivity(f, species.molar_mass, sc.diffusivity, D, f.v(0), f.v(1), powers, x);
+  else
+    throw ast::Error("transport device function: which is 1, 2 or 3");
+  return f.finish(out);
+}
+
 // Transport section of the per-state program: physical λ, μ and D_k = ρD_k/ρ with ρ = Σ_k W_k C_k.
 std::vector<E> transport_outputs(Block& f, const Species& s, const Polynomials& p, const TransportScales& scales, E N, E theta, bool lean,
                                  const std::vector<E>& x, E& mean_molar_massN) {

@@ -63,6 +63,7 @@ SIGNATURES = {
     "cb_program_from_linear": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_uint8), _I32P, _I32P, _I32P, _I32P, _DP, C.c_int32, _I32P,
                                          C.POINTER(_P)]),
     "cb_program_hip_source": (C.c_int, [_P, C.c_int32, C.c_uint32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cb_export_device_function": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "cb_hip_assemble": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
     "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
@@ -183,6 +184,26 @@ class Mechanism:
         h = _P()
         _check(lib().cb_build_transport(self._h, D, T0, viscosity, conductivity, C.byref(h)))
         return Program(h, self)
+
+    EXPORT_FILES = {"rates": 0, "conductivity": 1, "viscosity": 2, "diffusivity": 3}
+
+    def device_function(self, which, prefix="nekrk_", D=5):
+        """cb_export_device_function: the text of one file of the nekRK device-function export (cuda.rs:319-337);
+        which is "rates", "conductivity", "viscosity" or "diffusivity"."""
+        return _text(lib().cb_export_device_function, self._h, self.EXPORT_FILES[which], D, prefix.encode())
+
+    def export_device_functions(self, directory, prefix="nekrk_", D=5):
+        """Writes rates.c, conductivity.c, viscosity.c and diffusivity.c into directory (the reference writes them
+        to <target>/<mechanism>/, cuda.rs:319-320); returns their paths."""
+        import os
+        os.makedirs(directory, exist_ok=True)
+        paths = []
+        for which in self.EXPORT_FILES:
+            path = os.path.join(directory, which + ".c")
+            with open(path, "w") as f:
+                f.write(self.device_function(which, prefix, D))
+            paths.append(path)
+        return paths
 
     def chemistry(self, flags=RATES | TRANSPORT, D=4, T0=1000.0, viscosity=0.0, conductivity=0.0):
         """Per-state SoA program: inputs [T, P, Y_0..Y_{K-1}] -> [ω_0..ω_{Ka-1}, λ, μ, D_0..D_{K-1}, (dT/dt)]."""

@@ -96,6 +96,17 @@ int cb_program_from_linear(int32_t n_inputs, int64_t n_instructions, const uint8
 /* HIP text of a form (cuda.rs:101 compile counterpart). */
 int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed);
 
+/* Device-function export, the nekRK files of cuda.rs:140-350 (rates.c, conductivity.c, viscosity.c, diffusivity.c):
+ * per-state `__NEKRK_DEVICE__` functions in `dfloat`, named <prefix>species_rates(pressure_R, lnT, T, T2, T3, T4,
+ * rcpT, concentrations[], out[]) (out[k] = ω_k), <prefix>conductivityNIVT(sum_mole_proportions, lnT, lnT2..,
+ * mole_proportions[]) and <prefix>viscosityIVT(lnT, lnT2.., mole_proportions[]) (returned), and
+ * <prefix>density_diffusivity(id, mean_molar_mass_VTN, VT, lnT, lnT2.., mole_proportions[], out[], n_states)
+ * (out[k*n_states+id]); lnT^m are powers of ln(T/T0), and the transport coefficients are fits of length D scaled by
+ * cb_transport_scales (transport.rs:180-182). */
+enum cb_export { CB_EXPORT_RATES = 0, CB_EXPORT_CONDUCTIVITY = 1, CB_EXPORT_VISCOSITY = 2, CB_EXPORT_DIFFUSIVITY = 3 };
+int cb_export_device_function(const cb_mechanism* m, int32_t which, int32_t D, const char* prefix, char* buffer, size_t capacity,
+                               size_t* needed);
+
 /* device.rs:75 assemble: emit + hiprtc(gfx950) + module load. block_size 0 picks the form's default. */
 int cb_hip_assemble(const cb_program* p, int32_t form, uint32_t block_size, cb_module** out);
 /* Compiles a form into the code-object cache without touching a GPU. */

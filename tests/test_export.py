@@ -1,0 +1,95 @@
+"""The device-function export (the nekRK files of cuda.rs:140-350) on the CPU: the four exported functions are
+compiled as C++ with `dfloat` = double and evaluated per state, and must agree with the FP64 oracle to 1e-12 like
+the GPU forms (ω relative to the term scale, transport relative)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from combustion_amd import synth
+from helpers import relative_error, setup
+
+KB_NA = 1.380649e-23 * 6.02214076e23
+D = 5
+
+HARNESS = r"""
+#include <cmath>
+using std::exp; using std::log; using std::sqrt; using std::fmin; using std::fmax;
+#define dfloat double
+#define __NEKRK_DEVICE__ static
+%s
+extern "C" void rates(const double* s, const double* C, double* out) { nekrk_species_rates(s[0], s[1], s[2], s[3], s[4], s[5], s[6], C, out); }
+extern "C" double conductivity(double N, const double* l, const double* x) { return nekrk_conductivityNIVT(N, l[0], l[1], l[2], l[3], x); }
+extern "C" double viscosity(const double* l, const double* x) { return nekrk_viscosityIVT(l[0], l[1], l[2], l[3], x); }
+extern "C" void diffusivity(double W_VTN, double VT, const double* l, const double* x, double* out) {
+  nekrk_density_diffusivity(0, W_VTN, VT, l[0], l[1], l[2], l[3], x, out, 1);
+}
+"""
+
+
+@pytest.fixture(scope="module", params=["h2o2-10", "gri-53"])
+def exported(request, tmp_path_factory):
+    name = request.param
+    M, mech, o = setup(name, D=D)
+    d = tmp_path_factory.mktemp("export_" + name)
+    files = M.export_device_functions(str(d))
+    assert [os.path.basename(f) for f in files] == ["rates.c", "conductivity.c", "viscosity.c", "diffusivity.c"]
+    src = d / "harness.cpp"
+    src.write_text(HARNESS % "\n".join(open(f).read() for f in files))
+    lib = d / "libexport.so"
+    subprocess.run(["g++", "-O1", "-shared", "-fPIC", "-ffp-contract=off", "-o", str(lib), str(src)], check=True)
+    return name, M, o, C.CDLL(str(lib))
+
+
+def test_export_signatures_follow_the_reference(exported):
+    name, M, o, lib = exported
+    assert M.device_function("rates").startswith(
+        "__NEKRK_DEVICE__ void nekrk_species_rates(dfloat pressure_R, dfloat lnT, dfloat T, dfloat T2, dfloat T3, dfloat T4, "
+        "dfloat rcpT, const dfloat concentrations[], dfloat out[])")
+    assert M.device_function("viscosity", prefix="p_").startswith(
+        "__NEKRK_DEVICE__ dfloat p_viscosityIVT(dfloat lnT, dfloat lnT2, dfloat lnT3, dfloat lnT4, const dfloat mole_proportions[])")
+    assert "out[2*n_states+id]" in M.device_function("diffusivity")
+
+
+def test_exported_functions_match_the_oracle(exported):
+    name, M, o, lib = exported
+    K, Ka = M.K, M.active
+    T, P, Y = synth.config_states(name, 64, 11)
+    ref, floor = o.batch(T, P, Y)
+    T0, eta0, lam0 = M.transport_scales()
+    W = M.molar_mass
+    dp = C.POINTER(C.c_double)
+    lib.conductivity.restype = C.c_double
+    lib.viscosity.restype = C.c_double
+    lib.conductivity.argtypes = [C.c_double, dp, dp]
+    lib.viscosity.argtypes = [dp, dp]
+    lib.diffusivity.argtypes = [C.c_double, C.c_double, dp, dp, dp]
+    lib.rates.argtypes = [dp, dp, dp]
+    got = np.zeros_like(ref)
+    for s in range(len(T)):
+        # the per-state inputs as the chemistry program forms them: C_k = max(0, Y_k/W_k)/V, the bulk closing P/(R T)
+        pR = P[s] * (1 / KB_NA)
+        rcpT = 1 / T[s]
+        n = Y[:, s] * (1 / W)
+        density = pR * rcpT / n.sum()
+        Cn = density * np.maximum(0.0, n[:-1])
+        Ctot = pR * rcpT
+        x = np.append(Cn, Ctot - Cn.sum())
+        scal = np.array([pR, np.log(T[s]), T[s], T[s] ** 2, T[s] ** 3, T[s] ** 4, rcpT])
+        w = np.zeros(Ka)
+        lib.rates(scal.ctypes.data_as(dp), x.ctypes.data_as(dp), w.ctypes.data_as(dp))
+        theta = T[s] / T0
+        l = np.log(theta)
+        lp = np.array([l, l ** 2, l ** 3, l ** 4])
+        VT = np.sqrt(theta)
+        WN = (W * x).sum()
+        lam = lam0 * VT * lib.conductivity(Ctot, lp.ctypes.data_as(dp), x.ctypes.data_as(dp))
+        mu = eta0 * VT * lib.viscosity(lp.ctypes.data_as(dp), x.ctypes.data_as(dp))
+        dk = np.zeros(K)
+        lib.diffusivity(WN * VT, VT, lp.ctypes.data_as(dp), x.ctypes.data_as(dp), dk.ctypes.data_as(dp))
+        got[:, s] = np.concatenate([w, [lam, mu], dk * (eta0 / WN)])
+    err = relative_error(got, ref, floor)
+    print(name, "max error ω", err[:Ka].max(), "transport", err[Ka:].max(), flush=True)
+    assert err.max() < 1e-12
