@@ -194,8 +194,27 @@ Ratio forward_rate_constant(Block& f, const Reaction& r, const Temperatures& T, 
       E F = f.exp(f.div(lnFcent, f.add(f.mul(f1, f1), f.c(1.))));
       return {f.mul(f.mul(F, k_inf_e), Pr), f.add(Pr, f.c(1.))};
     }
-    case ReactionModel::SRI:
-      throw ast::Error("SRI falloff is not implemented (reaction.rs:194-203): " + r.equation);
+    case ReactionModel::SRI: {
+      // Not in the reference (reaction.rs:194-203 is unimplemented!): the SRI form of Stewart, Larson & Golden
+      // (1989) as Cantera documents it, F = d [a exp(-b/T) + exp(-T/c)]^X T^e with X = 1/(1 + log10(Pr)²)
+      // (exp(-T/c) dropped when c = 0), k = k∞ Pr/(1 + Pr) F, Pr = M k0/k∞ guarded like the Troe branch.
+      E M = efficiency(f, r.efficiencies, concentration, C);
+      E k0 = f.defv(arrhenius(f, r.k0, T), "k_0");
+      E k_inf = f.edef(arrhenius(f, r.rate_constant, T), "k_inf");
+      E Pr = f.defv(f.div(f.mul(M, k0), f.add(k_inf, f.c(flt_min))), "Pr");
+      E l = f.mul(f.ln(f.add(Pr, f.c(flt_min))), f.c(1. / std::log(10.)));
+      E X = f.defv(f.div(f.c(1.), f.add(f.c(1.), f.sq(l))), "X");
+      const SRI& s = r.sri;
+      std::vector<E> terms;
+      E first = f.exp(f.mul(f.c(-s.B), T.rcpT));
+      terms.push_back(s.A == 1. ? first : f.mul(f.c(s.A), first));
+      if (s.C != 0.) terms.push_back(f.exp(f.div(T.T, f.c(-s.C))));
+      E base = f.defv(f.sum(terms), "SRI base");
+      E F = f.exp(f.mul(X, f.ln(base)));
+      if (s.E != 0.) F = f.mul(F, f.exp(f.mul(f.c(s.E), T.lnT)));
+      if (s.D != 1.) F = f.mul(f.c(s.D), F);
+      return {f.mul(f.mul(f.defv(F, "F"), k_inf), Pr), f.add(Pr, f.c(1.))};
+    }
   }
   throw ast::Error("unknown reaction model");
 }

@@ -56,7 +56,7 @@ def lib():
 class _Mech(C.Structure):
     _fields_ = [("K", C.c_int), ("Ka", C.c_int), ("R", C.c_int), ("D", C.c_int)] + \
                [(n, C.c_void_p) for n in ("W", "split", "nasa", "reactants", "products", "model", "rate", "k0", "troe", "eff",
-                                          "fit_c", "fit_v", "fit_d")] + [("T0", C.c_double)]
+                                          "fit_c", "fit_v", "fit_d")] + [("T0", C.c_double), ("sri", C.c_void_p)]
 
 
 def _ptr(a):
@@ -74,6 +74,7 @@ class Oracle:
             "products": np.ascontiguousarray(mech["products"], np.int32), "model": np.ascontiguousarray(mech["model"], np.int32),
             "rate": np.ascontiguousarray(mech["rate"], np.float64), "k0": np.ascontiguousarray(mech["k0"], np.float64),
             "troe": np.ascontiguousarray(mech["troe"], np.float64), "eff": np.ascontiguousarray(mech["eff"], np.float64),
+            "sri": np.ascontiguousarray(mech["sri"], np.float64),
         }
         D = 0
         if fits is not None:
@@ -83,7 +84,8 @@ class Oracle:
                         fit_d=np.ascontiguousarray(d, np.float64))
         self._keep = keep
         self.s = _Mech(mech["K"], mech["Ka"], mech["R"], D, *[_ptr(keep.get(n)) for n in (
-            "W", "split", "nasa", "reactants", "products", "model", "rate", "k0", "troe", "eff", "fit_c", "fit_v", "fit_d")], T0)
+            "W", "split", "nasa", "reactants", "products", "model", "rate", "k0", "troe", "eff", "fit_c", "fit_v", "fit_d")], T0,
+            _ptr(keep["sri"]))
         self.has_transport = fits is not None
 
     def batch(self, T, P, Y, rates=True, transport=True, threads=1):

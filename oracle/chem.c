@@ -32,7 +32,7 @@ typedef struct {
   const double* nasa;       /* [K][2][7] */
   const int* reactants;     /* [R][K] */
   const int* products;      /* [R][K] */
-  const int* model;         /* [R] 0 elementary 1 irreversible 2 three-body 3 Lindemann 4 Troe */
+  const int* model;         /* [R] 0 elementary 1 irreversible 2 three-body 3 Lindemann 4 Troe 5 SRI */
   const double* rate;       /* [R][3] A (SI), b, Ea/R (K) */
   const double* k0;         /* [R][3] */
   const double* troe;       /* [R][4] A T3 T1 T2 */
@@ -41,6 +41,7 @@ typedef struct {
   const double* fit_v;      /* [K][D] √(η_k/√T) */
   const double* fit_d;      /* [K][K][D] P D_kj / T^1.5 (entries j<k used) */
   double T0;
+  const double* sri;        /* [R][5] a b c d e */
 } oracle_mechanism;
 
 /* Gibbs energy g = h/RT - s/R of species k (textbook NASA-7 integrals). */
@@ -116,6 +117,17 @@ void oracle_rates(const oracle_mechanism* m, double T, double P, const double* Y
     } else if (model == 3) {
       double k0 = arrhenius(m->k0 + 3 * r, T), kinf = arrhenius(m->rate + 3 * r, T);
       kf = M * k0 * kinf / (M * k0 + kinf + (double)FLT_MIN);
+    } else if (model == 5) {
+      /* SRI falloff (Stewart, Larson & Golden 1989, the form Cantera documents for its SRI rate):
+         F = d [a exp(-b/T) + exp(-T/c)]^X T^e, X = 1/(1 + log10(Pr)^2); exp(-T/c) is dropped when c = 0 */
+      double k0 = arrhenius(m->k0 + 3 * r, T), kinf = arrhenius(m->rate + 3 * r, T);
+      double Pr = M * k0 / (kinf + (double)FLT_MIN);
+      const double* s = m->sri + 5 * r;
+      double l = log10(Pr + (double)FLT_MIN);
+      double X = 1. / (1. + l * l);
+      double base = s[0] * exp(-s[1] / T) + (s[2] != 0. ? exp(-T / s[2]) : 0.);
+      double F = s[3] * pow(base, X) * pow(T, s[4]);
+      kf = kinf * Pr / (1. + Pr) * F;
     } else {
       const double Ainf = m->rate[3 * r];
       double k0n[3] = {m->k0[3 * r] / Ainf, m->k0[3 * r + 1], m->k0[3 * r + 2]};

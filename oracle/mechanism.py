@@ -65,7 +65,7 @@ def load(path=None, text=None):
             return [float(k["A"]) * math.pow(1e-6, exponent), float(k["b"]), Ea if ea_unit == "K" else Ea * 4.184 / R]
 
         rc = x.get("rate-constant") or x.get("high-P-rate-constant") or x["rate-constants"][0]
-        entry = {"eq": eq, "rate": rate(rc, order - 1), "k0": [0.0, 0.0, 0.0], "troe": [0.0, 0.0, 0.0, 0.0],
+        entry = {"eq": eq, "rate": rate(rc, order - 1), "k0": [0.0, 0.0, 0.0], "troe": [0.0, 0.0, 0.0, 0.0], "sri": [0.0] * 5,
                  "eff": {k: float(v) for k, v in (x.get("efficiencies") or {}).items()}}
         reversible = " <=> " in x["equation"]
         if kind in (None, "elementary", "pressure-dependent-Arrhenius"):
@@ -79,7 +79,9 @@ def load(path=None, text=None):
                 entry["model"] = "troe"
                 entry["troe"] = [float(t["A"]), float(t["T3"]), float(t["T1"]), float(t.get("T2", math.inf))]
             elif "SRI" in x:
+                s = x["SRI"]
                 entry["model"] = "sri"
+                entry["sri"] = [float(s["A"]), float(s["B"]), float(s["C"]), float(s.get("D", 1.0)), float(s.get("E", 0.0))]
             else:
                 entry["model"] = "pressure-modification"
         else:
@@ -115,7 +117,7 @@ def load(path=None, text=None):
         "reactants": reactants, "products": products,
         "model": np.array([MODELS[e["model"]] for e in reactions], np.int32),
         "rate": np.array([e["rate"] for e in reactions]), "k0": np.array([e["k0"] for e in reactions]),
-        "troe": np.array([e["troe"] for e in reactions]), "eff": eff,
+        "troe": np.array([e["troe"] for e in reactions]), "sri": np.array([e["sri"] for e in reactions]), "eff": eff,
         "geometry": geometry,
         "diameter": np.array([t["diameter"] * 1e-10 for t in tr]),
         "well_depth_J": np.array([t["well-depth"] * KB for t in tr]),

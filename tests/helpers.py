@@ -33,3 +33,34 @@ def state_floor(scale, Ka, ratio=1e-30):
     if Ka:
         f[:Ka] = np.maximum(scale[:Ka], ratio * scale[:Ka].max(axis=0, keepdims=True))
     return f
+
+
+def sri_variant(name, seed=5, params=None):
+    """YAML text of a named synthetic mechanism with every Troe block replaced by an SRI block: drawn
+    (a ~ U[0.5, 2], b ~ U[100, 1000] K, c ~ U[500, 3000] K, d ~ U[0.8, 1.2], e ~ U[-0.5, 0.5]; the first one with
+    c = 0), or the given (a, b, c, d, e) for all."""
+    import re
+    text = open(synth.mechanism_path(name)).read()
+    rng = np.random.default_rng(seed)
+    count = [0]
+
+    def block(_):
+        if params is not None:
+            a, b, c, d, e = params
+        else:
+            a, b, c, d, e = rng.uniform(0.5, 2), rng.uniform(100, 1000), rng.uniform(500, 3000), rng.uniform(0.8, 1.2), rng.uniform(-0.5, 0.5)
+            if count[0] == 0:
+                c = 0.0
+        count[0] += 1
+        return f"SRI: {{A: {a!r}, B: {b!r}, C: {c!r}, D: {d!r}, E: {e!r}}}"
+    return re.sub(r"Troe: \{[^}]*\}", block, text)
+
+
+@functools.lru_cache(maxsize=None)
+def setup_text(text, D=4):
+    """setup() for a mechanism given as YAML text."""
+    M = Mechanism(text=text)
+    mech = om.load(text=text)
+    fits = M.transport_fits(D, 1000.0)
+    T0, _, _ = M.transport_scales()
+    return M, mech, oracle.Oracle(mech, fits, T0)

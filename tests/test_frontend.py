@@ -221,14 +221,13 @@ def test_malformed_files_report_parse_errors(text, kind):
     assert e.value.kind == kind
 
 
-def test_sri_falloff_is_reported_unsupported():
-    """reaction.rs:194-203 leaves SRI unimplemented; the builders report it instead of aborting."""
+def test_sri_falloff_is_parsed_and_built():
+    """reaction.rs:194-203 leaves SRI unimplemented; this build parses it (yaml.rs:140-146) and builds it
+    (tests/test_sri.py checks the values)."""
     text = handwritten().replace("  Troe: {A: 0.7346, T3: 94.0, T1: 1756.0}", "  SRI: {A: 1.1, B: 700.0, C: 1234.0}")
     M = Mechanism(text=text)
     assert M.tables["reactions"][3]["model"] == "sri"
-    with pytest.raises(Error) as e:
-        M.chemistry()
-    assert e.value.kind == "unsupported"
+    assert M.chemistry().n_outputs == M.active + 2 + M.K
 
 
 def test_unknown_species_in_equation():
