@@ -201,6 +201,16 @@ def main():
                      "sections": sections},
     }
     forms = {"fp64": {"states_per_s": value, "kernel_ms": step_ms}}
+    # The same form with host buffers (the boundary's blocking host-pointer call: copies in, kernels, copies out).
+    host_in = [np.ascontiguousarray(x) for x in [T, P, *Y]]
+    host_out = [np.empty(N) for _ in range(prog.n_outputs)]
+    fp64(host_in, host_out)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        fp64(host_in, host_out)
+    host_s = (time.perf_counter() - t0) / 3
+    result["pcie_inclusive"] = {"states_per_s": N / host_s, "ms_per_step": host_s * 1e3,
+                                "note": "host numpy buffers through cb_hip_call (pageable memory, blocking); not the value"}
 
     for form in ("mixed", "fast"):
         mod = prog.assemble(form=form)

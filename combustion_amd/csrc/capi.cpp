@@ -130,6 +130,14 @@ std::string mechanism_json(const Mechanism& m) {
     rc("rate_constant", x.rate_constant);
     rc("k0", x.k0);
     o << ",\"troe\":[" << num(x.troe.A) << "," << num(x.troe.T3) << "," << num(x.troe.T1) << "," << num(x.troe.T2) << "]";
+    o << ",\"sri\":[" << num(x.sri.A) << "," << num(x.sri.B) << "," << num(x.sri.C) << "," << num(x.sri.D) << "," << num(x.sri.E) << "]";
+    o << ",\"plog\":[";
+    for (size_t i = 0; i < x.plog.size(); i++) {
+      const RateConstant& k = x.plog[i].k;
+      o << (i ? ",[" : "[") << num(x.plog[i].P) << "," << num(k.preexponential_factor) << "," << num(k.temperature_exponent) << ","
+        << num(k.activation_temperature) << "]";
+    }
+    o << "]";
     o << ",\"efficiencies\":[";
     for (size_t k = 0; k < x.efficiencies.size(); k++) o << (k ? "," : "") << num(x.efficiencies[k]);
     o << "]}";

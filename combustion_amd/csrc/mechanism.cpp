@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <map>
+#include <sstream>
 #include <stdexcept>
 
 #include "yaml.hpp"
@@ -131,7 +132,25 @@ struct RawReaction {
   RateConstant k0{};
   Troe troe{};
   SRI sri{};
+  std::vector<PlogEntry> plog;
 };
+
+// A pressure: a number in Pa, or a number followed by Pa, kPa, MPa, bar, atm or torr.
+double pressure_Pa(const Node& n, const std::string& where) {
+  if (n.kind != Node::Scalar) fail(where + ": missing pressure");
+  std::istringstream in(n.scalar);
+  double v;
+  std::string unit;
+  if (!(in >> v)) fail(where + ": bad pressure '" + n.scalar + "'");
+  in >> unit;
+  if (unit.empty() || unit == "Pa") return v;
+  if (unit == "kPa") return v * 1e3;
+  if (unit == "MPa") return v * 1e6;
+  if (unit == "bar") return v * 1e5;
+  if (unit == "atm") return v * 101325.;
+  if (unit == "torr") return v * (101325. / 760.);
+  fail(where + ": unknown pressure unit '" + unit + "'");
+}
 
 Element element_from(const std::string& s) {
   for (int e = 0; e < n_elements; e++)
@@ -227,17 +246,24 @@ Mechanism parse_yaml(const std::string& text) {
     const Node* k = &n["rate-constant"];
     if (k->is_null()) k = &n["high-P-rate-constant"];
     if (k->is_null()) {
-      if (type_name != "pressure-dependent-Arrhenius" || n["rate-constants"].size() != 1)
+      if (type_name != "pressure-dependent-Arrhenius" || n["rate-constants"].size() < 1)
         fail("reaction '" + r.text + "': no rate constant");
       k = &n["rate-constants"][0];
     }
     r.rate_constant = rate_constant(*k, reactants - 1);
+    if (type_name == "pressure-dependent-Arrhenius" && n["rate-constants"].size() > 1) {
+      // Not in the reference (yaml.rs:120,149 assert a single entry): Cantera's PLOG table.
+      for (size_t i = 0; i < n["rate-constants"].size(); i++) {
+        const Node& e = n["rate-constants"][i];
+        r.plog.push_back({pressure_Pa(e["P"], "reaction '" + r.text + "'"), rate_constant(e, reactants - 1)});
+      }
+      std::stable_sort(r.plog.begin(), r.plog.end(), [](const PlogEntry& a, const PlogEntry& b) { return a.P < b.P; });
+      if (r.plog.front().P <= 0) fail("reaction '" + r.text + "': non-positive PLOG pressure");
+    }
     for (auto& [name, v] : n["efficiencies"].pairs) r.efficiencies.emplace_back(name, v.f64());
     const bool reversible = r.text.find(" <=> ") != std::string::npos;
     const bool irreversible = !reversible && r.text.find(" => ") != std::string::npos;
     if (type_name.empty() || type_name == "elementary" || type_name == "pressure-dependent-Arrhenius") {
-      if (type_name == "pressure-dependent-Arrhenius" && n["rate-constants"].size() != 1)
-        fail("reaction '" + r.text + "': only single-entry pressure-dependent-Arrhenius is supported");
       if (reversible)
         r.model = ReactionModel::Elementary;
       else if (irreversible)
@@ -343,6 +369,7 @@ Mechanism parse_yaml(const std::string& text) {
     x.k0 = r.k0;
     x.troe = r.troe;
     x.sri = r.sri;
+    x.plog = r.plog;
     m.reactions.push_back(std::move(x));
   }
 

@@ -39,6 +39,11 @@ struct RateConstant {
 };
 struct Troe { double A, T3, T1, T2; };
 struct SRI { double A, B, C, D, E; };
+// One entry of a pressure-dependent Arrhenius (PLOG) table: the rate constant at pressure P (Pa).
+struct PlogEntry {
+  double P;
+  RateConstant k;
+};
 
 // lib.rs:84-91
 enum class ReactionModel { Elementary, Irreversible, ThreeBody, PressureModification, Troe, SRI };
@@ -55,6 +60,9 @@ struct Reaction {
   RateConstant k0{};                 // low-pressure limit (falloff models)
   Troe troe{};
   SRI sri{};
+  // Pressure-dependent Arrhenius with more than one entry (Elementary/Irreversible models), sorted by P; entries
+  // of equal P are summed. Empty for the single-entry form, which is rate_constant itself (yaml.rs:120,149).
+  std::vector<PlogEntry> plog;
 };
 
 enum class Geometry { Atom, Linear, Nonlinear };

@@ -132,6 +132,54 @@ E arrhenius(Block& f, const RateConstant& k, const Temperatures& T) {
   return f.min(f.c(flt_max), k_);
 }
 
+// Pressure-dependent Arrhenius with several entries (not in the reference, whose yaml.rs:120,149 assert one):
+// Cantera's PLOG rate, ln k interpolated linearly in ln P between the two table pressures around P, and the first
+// or last entry's k beyond the table; entries of equal pressure are summed. ln P = ln(P/(R T)) + ln T + ln R from
+// the state's total concentration. The result is clamped at FLT_MAX like every Arrhenius rate (reaction.rs:138).
+E plog_rate(Block& f, const Reaction& r, const Temperatures& T, E concentration) {
+  std::vector<std::pair<double, std::vector<RateConstant>>> levels;
+  for (const PlogEntry& e : r.plog) {
+    if (levels.empty() || levels.back().first != e.P) levels.push_back({e.P, {}});
+    levels.back().second.push_back(e.k);
+  }
+  std::vector<E> lnk;
+  for (auto& [P, ks] : levels) {
+    if (ks.size() == 1 && ks[0].preexponential_factor > 0) {
+      const RateConstant& k = ks[0];
+      std::vector<E> t = {f.c(std::log(k.preexponential_factor))};
+      if (k.temperature_exponent != 0.) t.push_back(f.mul(f.c(k.temperature_exponent), T.lnT));
+      if (k.activation_temperature != 0.) t.push_back(f.mul(f.c(-k.activation_temperature), T.rcpT));
+      lnk.push_back(f.defv(f.sum(t), "ln k(P)"));
+    } else {
+      std::vector<E> t;
+      for (const RateConstant& k : ks)
+        t.push_back(f.mul(f.c(k.preexponential_factor),
+                          f.exp(f.sub(f.mul(f.c(k.temperature_exponent), T.lnT), f.mul(f.c(k.activation_temperature), T.rcpT)))));
+      lnk.push_back(f.defv(f.ln(f.sum(t)), "ln k(P)"));
+    }
+  }
+  E v = lnk.back();
+  if (levels.size() > 1) {
+    E lnP = f.defv(f.add(f.add(f.ln(concentration), T.lnT), f.c(std::log(R))), "lnP");
+    // from the top: v = ln k beyond the table, then each interval (ln P_i, ln P_{i+1}] and below the table
+    for (int i = (int)levels.size() - 2; i >= -1; i--) {
+      E t;
+      if (i >= 0) {
+        const double lo = std::log(levels[i].first), hi = std::log(levels[i + 1].first);
+        t = f.add(lnk[i], f.mul(f.sub(lnk[i + 1], lnk[i]), f.mul(f.sub(lnP, f.c(lo)), f.c(1. / (hi - lo)))));
+      } else {
+        t = lnk[0];
+      }
+      // lnP <= ln P_{i+1}: below the interval's top (i = -1: below the table)
+      const double edge = std::log(levels[i + 1].first);
+      int32_t id = f.value("ln k(P)");
+      f.select(f.le(lnP, f.c(edge)), {t}, {v}, {id});
+      v = f.v(id);
+    }
+  }
+  return f.min(f.c(flt_max), f.exp(v));
+}
+
 // reaction.rs:141-145
 E efficiency(Block& f, const std::vector<double>& efficiencies, E concentration, const std::vector<E>& C) {
   std::vector<E> non_one;
@@ -154,6 +202,7 @@ Ratio forward_rate_constant(Block& f, const Reaction& r, const Temperatures& T, 
   switch (r.model) {
     case ReactionModel::Elementary:
     case ReactionModel::Irreversible:
+      if (!r.plog.empty()) return {f.edef(plog_rate(f, r, T, concentration), "k"), std::nullopt};
       return {f.edef(arrhenius(f, r.rate_constant, T), "k"), std::nullopt};
     case ReactionModel::ThreeBody:
       return {f.mul(arrhenius(f, r.rate_constant, T), efficiency(f, r.efficiencies, concentration, C)), std::nullopt};

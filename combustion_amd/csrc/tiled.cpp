@@ -45,7 +45,7 @@ int model_rank(ReactionModel m) {
     case ReactionModel::ThreeBody: return 1;
     case ReactionModel::PressureModification: return 2;
     case ReactionModel::Troe: return 3;
-    case ReactionModel::SRI: throw Unsupported("SRI falloff is not implemented (reaction.rs:194-203)");
+    case ReactionModel::SRI: throw Unsupported("the MFMA rates kernel has no SRI falloff model (the other forms do)");
   }
   return 0;
 }
@@ -65,6 +65,8 @@ MixedTables mixed_tables(const Mechanism& m) {
   // model (the kernel then branches on the half, not on the row); then padded to whole 32-row tiles.
   std::vector<int> order(R);
   for (int r = 0; r < R; r++) order[r] = r;
+  for (const Reaction& r : m.reactions)
+    if (!r.plog.empty()) throw Unsupported("the MFMA rates kernel has no pressure-dependent Arrhenius tables (the other forms do)");
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return model_rank(m.reactions[a].model) < model_rank(m.reactions[b].model); });
   std::vector<int> rows;  // padded row -> reaction (or -1)
   for (size_t i = 0; i < order.size();) {

@@ -56,7 +56,7 @@ def lib():
 class _Mech(C.Structure):
     _fields_ = [("K", C.c_int), ("Ka", C.c_int), ("R", C.c_int), ("D", C.c_int)] + \
                [(n, C.c_void_p) for n in ("W", "split", "nasa", "reactants", "products", "model", "rate", "k0", "troe", "eff",
-                                          "fit_c", "fit_v", "fit_d")] + [("T0", C.c_double), ("sri", C.c_void_p)]
+                                          "fit_c", "fit_v", "fit_d")] + [("T0", C.c_double), ("sri", C.c_void_p), ("plog_offset", C.c_void_p), ("plog", C.c_void_p)]
 
 
 def _ptr(a):
@@ -75,6 +75,8 @@ class Oracle:
             "rate": np.ascontiguousarray(mech["rate"], np.float64), "k0": np.ascontiguousarray(mech["k0"], np.float64),
             "troe": np.ascontiguousarray(mech["troe"], np.float64), "eff": np.ascontiguousarray(mech["eff"], np.float64),
             "sri": np.ascontiguousarray(mech["sri"], np.float64),
+            "plog_offset": np.ascontiguousarray(np.cumsum([0] + [len(p) for p in mech["plog"]]), np.int32),
+            "plog": np.ascontiguousarray(np.array([row for p in mech["plog"] for row in p] or [[0.0] * 4], np.float64)),
         }
         D = 0
         if fits is not None:
@@ -85,7 +87,7 @@ class Oracle:
         self._keep = keep
         self.s = _Mech(mech["K"], mech["Ka"], mech["R"], D, *[_ptr(keep.get(n)) for n in (
             "W", "split", "nasa", "reactants", "products", "model", "rate", "k0", "troe", "eff", "fit_c", "fit_v", "fit_d")], T0,
-            _ptr(keep["sri"]))
+            _ptr(keep["sri"]), _ptr(keep["plog_offset"]), _ptr(keep["plog"]))
         self.has_transport = fits is not None
 
     def batch(self, T, P, Y, rates=True, transport=True, threads=1):

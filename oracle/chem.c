@@ -42,7 +42,31 @@ typedef struct {
   const double* fit_d;      /* [K][K][D] P D_kj / T^1.5 (entries j<k used) */
   double T0;
   const double* sri;        /* [R][5] a b c d e */
+  const int* plog_offset;   /* [R+1] rows of plog per reaction (more than one: a PLOG table) */
+  const double* plog;       /* [rows][4] P (Pa), A (SI), b, Ea/R (K); sorted by P within a reaction */
 } oracle_mechanism;
+
+/* Cantera's PLOG rate (not in the reference): k at each table pressure (entries of equal P summed), ln k linear in
+   ln P between the two table pressures around P, the end values beyond the table; clamped at FLT_MAX. */
+static double plog_rate(const double* e, int n, double T, double P) {
+  double Pl[64], kl[64];
+  int L = 0;
+  for (int i = 0; i < n; i++) {
+    double k = e[4 * i + 1] * pow(T, e[4 * i + 2]) * exp(-e[4 * i + 3] / T);
+    if (L && Pl[L - 1] == e[4 * i]) kl[L - 1] += k;
+    else { Pl[L] = e[4 * i]; kl[L] = k; L++; }
+  }
+  double k;
+  if (P <= Pl[0]) k = kl[0];
+  else if (P > Pl[L - 1]) k = kl[L - 1];
+  else {
+    int i = 0;
+    while (!(P <= Pl[i + 1])) i++;
+    double lk = log(kl[i]) + (log(kl[i + 1]) - log(kl[i])) * (log(P) - log(Pl[i])) / (log(Pl[i + 1]) - log(Pl[i]));
+    k = exp(lk);
+  }
+  return fmin((double)FLT_MAX, k);
+}
 
 /* Gibbs energy g = h/RT - s/R of species k (textbook NASA-7 integrals). */
 static double gibbs_RT(const oracle_mechanism* m, int k, double T) {
@@ -110,7 +134,10 @@ void oracle_rates(const oracle_mechanism* m, double T, double P, const double* Y
     double M = C_tot;
     if (model >= 2)
       for (int k = 0; k < K; k++) M += (m->eff[(size_t)r * K + k] - 1.) * C[k];
-    if (model <= 1) {
+    const int n_plog = m->plog_offset ? m->plog_offset[r + 1] - m->plog_offset[r] : 0;
+    if (model <= 1 && n_plog > 1) {
+      kf = plog_rate(m->plog + 4 * m->plog_offset[r], n_plog, T, C_tot * RGAS * T);
+    } else if (model <= 1) {
       kf = arrhenius(m->rate + 3 * r, T);
     } else if (model == 2) {
       kf = arrhenius(m->rate + 3 * r, T) * M;

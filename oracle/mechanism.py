@@ -40,6 +40,14 @@ def _equation(text):
     return out
 
 
+def _pressure(p):
+    """A YAML pressure: a number (Pa) or '<number> <unit>' with unit Pa, kPa, MPa, bar, atm or torr."""
+    if isinstance(p, (int, float)):
+        return float(p)
+    value, unit = (str(p).split() + ["Pa"])[:2]
+    return float(value) * {"Pa": 1.0, "kPa": 1e3, "MPa": 1e6, "bar": 1e5, "atm": 101325.0, "torr": 101325.0 / 760.0}[unit]
+
+
 def load(path=None, text=None):
     """Flat arrays of a mechanism (SI units), species in the reference's active-first order."""
     doc = yaml.safe_load(text if text is not None else open(path).read())
@@ -65,7 +73,10 @@ def load(path=None, text=None):
             return [float(k["A"]) * math.pow(1e-6, exponent), float(k["b"]), Ea if ea_unit == "K" else Ea * 4.184 / R]
 
         rc = x.get("rate-constant") or x.get("high-P-rate-constant") or x["rate-constants"][0]
-        entry = {"eq": eq, "rate": rate(rc, order - 1), "k0": [0.0, 0.0, 0.0], "troe": [0.0, 0.0, 0.0, 0.0], "sri": [0.0] * 5,
+        plog = []
+        if kind == "pressure-dependent-Arrhenius" and len(x["rate-constants"]) > 1:
+            plog = sorted(([_pressure(e["P"])] + rate(e, order - 1) for e in x["rate-constants"]), key=lambda row: row[0])
+        entry = {"eq": eq, "rate": rate(rc, order - 1), "k0": [0.0, 0.0, 0.0], "troe": [0.0, 0.0, 0.0, 0.0], "sri": [0.0] * 5, "plog": plog,
                  "eff": {k: float(v) for k, v in (x.get("efficiencies") or {}).items()}}
         reversible = " <=> " in x["equation"]
         if kind in (None, "elementary", "pressure-dependent-Arrhenius"):
@@ -118,6 +129,7 @@ def load(path=None, text=None):
         "model": np.array([MODELS[e["model"]] for e in reactions], np.int32),
         "rate": np.array([e["rate"] for e in reactions]), "k0": np.array([e["k0"] for e in reactions]),
         "troe": np.array([e["troe"] for e in reactions]), "sri": np.array([e["sri"] for e in reactions]), "eff": eff,
+        "plog": [e["plog"] for e in reactions],
         "geometry": geometry,
         "diameter": np.array([t["diameter"] * 1e-10 for t in tr]),
         "well_depth_J": np.array([t["well-depth"] * KB for t in tr]),

@@ -64,3 +64,25 @@ def setup_text(text, D=4):
     fits = M.transport_fits(D, 1000.0)
     T0, _, _ = M.transport_scales()
     return M, mech, oracle.Oracle(mech, fits, T0)
+
+
+def plog_variant(name, every=3, seed=6):
+    """YAML text of a named synthetic mechanism with every `every`-th elementary reaction turned into a PLOG table:
+    entries at 2, 8 and 30 atm (its rate constant scaled by drawn factors and exponents), plus a second entry at
+    30 atm that the table sums."""
+    import re
+    text = open(synth.mechanism_path(name)).read()
+    rng = np.random.default_rng(seed)
+    count = [0]
+
+    def table(m):
+        count[0] += 1
+        if count[0] % every:
+            return m.group(0)
+        A, b, Ea = float(m.group(2)), float(m.group(3)), float(m.group(4))
+        rows = []
+        for P in ("2.0 atm", "8.0 atm", "30.0 atm", "30.0 atm"):
+            f, db = rng.uniform(0.2, 5.0), rng.uniform(-0.3, 0.3)
+            rows.append(f"  - {{P: {P}, A: {A * f!r}, b: {b + db!r}, Ea: {Ea * rng.uniform(0.8, 1.2)!r}}}")
+        return f"{m.group(1)}\n  type: pressure-dependent-Arrhenius\n  rate-constants:\n" + "\n".join(rows)
+    return re.sub(r"(- equation: [^\n]*<=>[^\n(]*)\n  rate-constant: \{A: ([^,]*), b: ([^,]*), Ea: ([^}]*)\}", table, text)
