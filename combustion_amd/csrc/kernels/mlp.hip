@@ -111,13 +111,22 @@ extern "C" __global__ void mlp_reduce(const float* __restrict__ partial, long n,
   out[i] = s;
 }
 
-// db[c] = Σ_m dz[m][c] over the rows of one column (BF16 gradients, FP32 sum), one thread per column.
-extern "C" __global__ void mlp_column_sum(const u16* __restrict__ dz, long M, long ld, long cols, float* __restrict__ out) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// Partial column sums of dz (BF16 gradients, FP32 sums) over the row chunk [z·chunk, (z+1)·chunk) of blockIdx.z:
+// a 256-thread block takes 64 columns (the lane) and 4 interleaved row streams (the wave), combined in LDS in wave
+// order, into partial[z][c]; mlp_reduce then adds the chunks in order (a fixed summation order for every run).
+extern "C" __global__ void __launch_bounds__(256)
+mlp_column_sum(const u16* __restrict__ dz, long M, long ld, long cols, long chunk, float* __restrict__ partial) {
+  __shared__ float part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long c = (long)blockIdx.x * 64 + lane;
+  const long m0 = (long)blockIdx.z * chunk, m1 = m0 + chunk < M ? m0 + chunk : M;
   float s = 0.f;
-  for (long m = 0; m < M; m++) s += from_bf16(dz[m * ld + c]);
-  out[c] = s;
+  if (c < cols)
+#pragma unroll 8
+    for (long m = m0 + wave; m < m1; m += 4) s += from_bf16(dz[m * ld + c]);
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && c < cols) partial[(long)blockIdx.z * cols + c] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 // x (FP32) -> BF16, elementwise.

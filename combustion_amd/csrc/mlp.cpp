@@ -130,10 +130,12 @@ void Mlp::backward_mse(const uint16_t* params16, const float* y, const float* ta
     float* gw = grad + L.w;
     void* a1[] = {&v.partial, &size, &p, &gw};
     device::check(hipModuleLaunchKernel(reduce_, blocks(size, 256), 1, 1, 256, 1, 1, 0, s, a1, nullptr), "launch mlp_reduce");
-    long out = L.out;
+    long out = L.out, chunk = kChunk;
     float* gb = grad + L.b;
-    void* a2[] = {&v.g[cur], &n, &out, &out, &gb};
-    device::check(hipModuleLaunchKernel(colsum_, blocks(out, 64), 1, 1, 64, 1, 1, 0, s, a2, nullptr), "launch mlp_column_sum");
+    void* a2[] = {&v.g[cur], &n, &out, &out, &chunk, &v.partial};
+    device::check(hipModuleLaunchKernel(colsum_, blocks(out, 64), 1, parts, 256, 1, 1, 0, s, a2, nullptr), "launch mlp_column_sum");
+    void* a3[] = {&v.partial, &out, &p, &gb};
+    device::check(hipModuleLaunchKernel(reduce_, blocks(out, 256), 1, 1, 256, 1, 1, 0, s, a3, nullptr), "launch mlp_reduce");
     if (l > 0) {
       // dZ_{l-1} = (dZ_l W_l) · SiLU'(Z_{l-1})
       gemm(2, n, L.in, L.out, v.g[cur], L.out, 1, params16 + L.w, 1, L.in, nullptr, v.z[l - 1], nullptr, v.g[1 - cur], L.in, nullptr, 1, s);
