@@ -34,9 +34,10 @@ typedef unsigned short u16;
 #define NB 64
 #define NF (K + 2)
 #define KW ((K + 3) / 4)  // species per wave in stage 1
-#define HS ((IN > WIDTH ? IN : WIDTH) + 8)  // activation row stride (BF16): an odd number of 16-byte units, so the
-                                             // 16 rows of a B fragment start in distinct bank groups
-static_assert(IN + 2 * (2 * NM + 1) <= HS, "stage-1 scratch does not fit the activation rows");
+// Activation row (BF16 elements): the inputs and the stage-1 scratch, or the hidden width, rounded up to 16 and plus 8,
+// an odd number of 16-byte units, so that the 16 rows of a B fragment start in distinct bank groups.
+#define ROW_NEED (IN + 2 * (2 * NM + 1) > WIDTH ? IN + 2 * (2 * NM + 1) : WIDTH)
+#define HS ((ROW_NEED + 15) / 16 * 16 + 8)
 
 __device__ __forceinline__ u16 to_bf16(float x) {
   unsigned u = __float_as_uint(x);

@@ -513,8 +513,12 @@ class Surrogate : public SoAForm {
     // for Wilke, r(k) for diffusion).
     const int b = K - 1, NM = (int)w.major.size();
     if (NM < 1 || NM > K) throw ArgumentError("the number of major species is 1..K");
-    if (lay.in_pad + 2 * (2 * NM + 1) > std::max(lay.in_pad, (long)lay.width) + 8)
-      throw Unsupported("too many major species for the surrogate kernel's stage-1 scratch");
+    {
+      // LDS of kernels/surrogate.hip: activation rows, outputs, partial sums
+      const long row = (std::max(lay.in_pad + 2 * (2 * NM + 1), (long)lay.width) + 15) / 16 * 16 + 8;
+      const long lds = 64 * row * 2 + (long)NF * 64 * 4 + 4 * 3 * 64 * 4 + 4 * 64 * 4;
+      if (lds > 160 * 1024) throw Unsupported("the surrogate kernel's LDS does not fit 160 KB for this mechanism and network");
+    }
     std::vector<int32_t> slot(K, -1);
     int other = -1;
     for (int i = 0; i < NM; i++) {
