@@ -66,9 +66,13 @@ TransportScales transport_scales(const Mechanism& m, double T0 = 1000.);
 // 1/V = (P/R)/T / Σ_j Y_j/W_j, so C_k = max(0, n_k)/V for k < K-1 and the bulk species closes C_tot = P/(R T).
 struct ChemistryOptions {
   bool rates = true, transport = true, rhs = false;
-  // Register-lean viscosity for the one-lane-per-state kernels: v_k is evaluated again (in Horner form) where
-  // Wilke's k-th row needs it instead of living beside 1/v_j through the whole sum; the same values to rounding.
-  bool lean_viscosity = false;
+  // Transport program shaped for the one-lane-per-state kernels; the same values to rounding (a few ulps per term).
+  // Register-lean viscosity: v_k is evaluated again (in Horner form) where Wilke's k-th row needs it instead of
+  // living beside 1/v_j through the whole sum. Issue-lean pair terms: each K×K term holds one per-pair constant,
+  // so that no instruction needs two (gfx950 instructions read at most one scalar register; a second constant costs
+  // two VGPR moves): Wilke's (a_kj + b_kj v_k/v_j)² as a_kj² (1 + ṽ_k/ṽ_j)² with ṽ_k = v_k W_k^(-1/4), and the
+  // binary-diffusion fit c0 + c1 ℓ + … as c0 (1 + (c1/c0) ℓ + …).
+  bool lean_transport = false;
 };
 ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
 

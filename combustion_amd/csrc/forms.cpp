@@ -40,12 +40,14 @@ class Straight : public SoAForm {
 }  // namespace
 
 // Form 0 keeps every operation as the FP64 interpreter rounds it; form 1 on a straight-line program allows FMA
-// contraction and reciprocal-multiply division (both within an ulp or two per operation); form 3 also evaluates
-// exp and ln with FP32 cores.
+// contraction, reciprocal-multiply division with one Newton step (relative error below 1e-14) and the polynomial
+// exponential (a few ulps), as the FP64 chemistry kernels do; form 3 also evaluates exp and ln with FP32 cores.
 StraightOptions straight_options(int form, unsigned block_size) {
   StraightOptions o;
   o.block_size = block_size ? block_size : 256;
   o.reciprocal = form == 1 || form == 3;
+  o.newton_steps = form == 1 ? 1 : 2;
+  o.exp_poly = form == 1;
   o.fast_transcendentals = form == 3;
   return o;
 }

@@ -21,14 +21,15 @@ std::string literal(double x) {
 std::string emit_straight(const ast::Linear& l, const std::string& name, const StraightOptions& o) {
   using ast::LOp;
   std::ostringstream s;
-  if (o.reciprocal && !o.f32)
+  if (o.reciprocal && !o.f32) {
     s << "__device__ __forceinline__ double rcp64(double b) {\n"
-         "  double r = __builtin_amdgcn_rcp(b);\n"
-         "  double e = __builtin_fma(-b, r, 1.0);\n"
-         "  r = __builtin_fma(r, e, r);\n"
-         "  e = __builtin_fma(-b, r, 1.0);\n"
-         "  return __builtin_fma(r, e, r);\n"
+         "  double r = __builtin_amdgcn_rcp(b);\n";
+    for (unsigned i = 0; i < o.newton_steps; i++)
+      s << "  { const double e = __builtin_fma(-b, r, 1.0);\n"
+           "    r = __builtin_fma(r, e, r); }\n";
+    s << "  return r;\n"
          "}\n";
+  }
   if (o.fast_transcendentals && !o.f32)
     s << "__device__ __forceinline__ double exp_wide(double x) {\n"
          "  const double y = x * 1.4426950408889634;\n"
@@ -40,6 +41,27 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
          "  const int e = (int)((b >> 52) & 0x7FF) - 1023;\n"
          "  const double m = __longlong_as_double((b & 0x000FFFFFFFFFFFFFLL) | 0x3FF0000000000000LL);\n"
          "  return ((double)e + (double)__builtin_amdgcn_logf((float)m)) * 0.6931471805599453;\n"
+         "}\n";
+  if (o.exp_poly && !o.f32 && !o.fast_transcendentals)
+    s << "__device__ __forceinline__ double exp64(double x) {\n"
+         "  x = __builtin_fmin(__builtin_fmax(x, -746.0), 710.0);\n"
+         "  const double t = __builtin_fma(x, 0x1.71547652b82fep+0, 0x1.8p+52);\n"
+         "  const double n = t - 0x1.8p+52;\n"
+         "  double r = __builtin_fma(-n, 0x1.62e42fee00000p-1, x);\n"
+         "  r = __builtin_fma(-n, 0x1.a39ef35793c76p-33, r);\n"
+         "  double p = 0x1.af631d0059becp-26;\n"
+         "  p = __builtin_fma(p, r, 0x1.28b4057f44145p-22);\n"
+         "  p = __builtin_fma(p, r, 0x1.71ddf5749d126p-19);\n"
+         "  p = __builtin_fma(p, r, 0x1.a01991ac8730ap-16);\n"
+         "  p = __builtin_fma(p, r, 0x1.a01a01b14378fp-13);\n"
+         "  p = __builtin_fma(p, r, 0x1.6c16c187fbe02p-10);\n"
+         "  p = __builtin_fma(p, r, 0x1.111111110f225p-7);\n"
+         "  p = __builtin_fma(p, r, 0x1.555555554f0cfp-5);\n"
+         "  p = __builtin_fma(p, r, 0x1.555555555555ap-3);\n"
+         "  p = __builtin_fma(p, r, 0x1.0000000000011p-1);\n"
+         "  p = __builtin_fma(p, r, 1.0);\n"
+         "  p = __builtin_fma(p, r, 1.0);\n"
+         "  return __builtin_ldexp(p, (int)(unsigned)__double_as_longlong(t));\n"
          "}\n";
   const char* F = o.f32 ? "float" : "double";
   if (o.f32)
@@ -85,7 +107,7 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
           s << a << " / " << b;
         break;
       case LOp::Sqrt: s << (o.f32 ? "__builtin_sqrtf(" : "sqrt(") << a << ")"; break;
-      case LOp::Exp: s << (o.f32 ? "exp32(" : o.fast_transcendentals ? "exp_wide(" : "exp(") << a << ")"; break;
+      case LOp::Exp: s << (o.f32 ? "exp32(" : o.fast_transcendentals ? "exp_wide(" : o.exp_poly ? "exp64(" : "exp(") << a << ")"; break;
       case LOp::Ln: s << (o.f32 ? "log32(" : o.fast_transcendentals ? "log_wide(" : "log(") << a << ")"; break;
       case LOp::Sq: s << a << " * " << a; break;
       case LOp::Select: s << a << " ? " << b << " : " << c; break;

@@ -495,7 +495,7 @@ ast::Function chemistry(const Mechanism& m, const Polynomials* poly, const Trans
   if (o.transport) {
     E theta = f.defv(f.mul(T, f.c(1. / scales.T0)), "T/T0");
     E W_N;
-    std::vector<E> tr = transport_outputs(f, m.species, *poly, scales, p.concentration, theta, o.lean_viscosity,
+    std::vector<E> tr = transport_outputs(f, m.species, *poly, scales, p.concentration, theta, o.lean_transport,
                                           std::vector<E>(p.C.begin(), p.C.end() - 1), W_N);
     out.insert(out.end(), tr.begin(), tr.end());
   }

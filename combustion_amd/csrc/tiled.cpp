@@ -332,7 +332,9 @@ std::string rates_straight_text(const ProgramView& p, bool fast) {
   o.transport = false;
   StraightOptions so;
   so.reciprocal = true;
+  so.newton_steps = 1;
   so.fast_transcendentals = fast;
+  so.exp_poly = true;
   return emit_straight(ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o)), fast ? "rates_fast" : "rates_fp64", so);
 }
 
@@ -405,9 +407,10 @@ bool straight_transport(const ProgramView& p) { return p.mechanism->species.len(
 std::string transport_straight_text(const ProgramView& p, bool f64) {
   ChemistryOptions o;
   o.rates = false;
-  o.lean_viscosity = true;
+  o.lean_transport = true;
   StraightOptions so;
   so.reciprocal = f64;
+  so.newton_steps = 1;
   so.f32 = !f64;
   // FP64: the lean viscosity program fits two waves per SIMD (255 VGPRs, no AGPRs), 30 % faster at K=53 than one
   // wave with 329 registers; the FP32 kernel already holds two waves (251 VGPRs) and a third one was slower.

@@ -292,8 +292,25 @@ E viscosityIVT(Block& f, const std::vector<double>& W, const std::vector<double>
   const size_t K = x.size();
   std::vector<E> v, r;
   if (lean && D >= 2) {
-    for (size_t k = 0; k < K; k++) r.push_back(f.defv(f.div(f.c(1.), dot_consts(f, &P[k * D], lnT)), "rcp_VviscosityIVVT" + std::to_string(k)));
-    for (size_t k = 0; k < K; k++) v.push_back(horner(f, &P[k * D], D, lnT));
+    // ṽ_k = v_k W_k^(-1/4): the fit's coefficients scaled at build time, so that Wilke's term for the pair (k, j)
+    // is a_kj² (1 + ṽ_k/ṽ_j)² with a_kj² = 1/√8 /√(1 + W_k/W_j), and v_k² = √W_k ṽ_k² in the numerator.
+    std::vector<double> Q(P.begin(), P.begin() + K * D);
+    for (size_t k = 0; k < K; k++)
+      for (int m = 0; m < D; m++) Q[k * D + m] *= 1. / std::sqrt(std::sqrt(W[k]));
+    for (size_t k = 0; k < K; k++) r.push_back(f.defv(f.div(f.c(1.), dot_consts(f, &Q[k * D], lnT)), "rcp_VviscosityIVVT" + std::to_string(k)));
+    for (size_t k = 0; k < K; k++) v.push_back(horner(f, &Q[k * D], D, lnT));
+    std::vector<E> outer;
+    for (size_t k = 0; k < K; k++) {
+      std::vector<E> inner;
+      for (size_t j = 0; j < K; j++) {
+        const double a2 = 1. / std::sqrt(8.) * 1. / std::sqrt(1. + W[k] / W[j]);
+        // x_j (1 + ṽ_k/ṽ_j)² first: x_j a_kj² would be one node shared by every row k of the same molar mass (CH2 and
+        // CH2(S), say), which would keep the K products of the first such row live until the last one.
+        inner.push_back(f.mul(f.mul(x[j], f.sq(f.add(f.c(1.), f.mul(v[k], r[j])))), f.c(a2)));
+      }
+      outer.push_back(f.div(f.mul(f.mul(x[k], f.c(std::sqrt(W[k]))), f.sq(v[k])), f.sum(inner)));
+    }
+    return f.sum(outer);
   } else {
     for (size_t k = 0; k < K; k++) v.push_back(f.defv(dot_consts(f, &P[k * D], lnT), "VviscosityIVVT" + std::to_string(k)));
     for (size_t k = 0; k < K; k++) r.push_back(f.defv(f.div(f.c(1.), v[k]), "rcp_VviscosityIVVT" + std::to_string(k)));
@@ -313,13 +330,21 @@ E viscosityIVT(Block& f, const std::vector<double>& W, const std::vector<double>
 
 // transport.rs:162-177
 std::vector<E> density_diffusivity(Block& f, const std::vector<double>& W, const std::vector<double>& P, int D, E mean_molar_mass_VTN, E VT,
-                                   const std::vector<E>& lnT, const std::vector<E>& x) {
+                                   const std::vector<E>& lnT, const std::vector<E>& x, bool lean = false) {
   const size_t K = x.size();
   std::vector<std::optional<E>> S(K);
   for (size_t k = 0; k < K; k++)
     for (size_t j = 0; j < k; j++) {
-      E y = dot_consts(f, &P[(k * K + j) * D], lnT);
-      E Rkj = f.defv(f.div(f.c(1.), y), "R" + std::to_string(k) + "_" + std::to_string(j));
+      const double* c = &P[(k * K + j) * D];
+      E Rkj;
+      if (lean && D >= 2 && c[0] != 0.) {
+        // 1/(c0 + c1 ℓ + …) as (1/c0)/(1 + (c1/c0) ℓ + …)
+        std::vector<E> terms = {f.c(1.)};
+        for (int m = 1; m < D; m++) terms.push_back(f.mul(f.c(c[m] / c[0]), lnT[m]));
+        Rkj = f.defv(f.div(f.c(1. / c[0]), f.sum(terms)), "R" + std::to_string(k) + "_" + std::to_string(j));
+      } else {
+        Rkj = f.defv(f.div(f.c(1.), dot_consts(f, c, lnT)), "R" + std::to_string(k) + "_" + std::to_string(j));
+      }
       E t = f.mul(x[j], Rkj);
       S[k] = f.defv(S[k] ? f.add(*S[k], t) : t, "S" + std::to_string(k) + "_" + std::to_string(j));
       t = f.mul(x[k], Rkj);
@@ -364,7 +389,7 @@ std::vector<E> properties_body(Block& f, const Species& species, const Polynomia
   std::vector<E> out;
   out.push_back(f.mul(conductivityNIVT(f, sc.conductivity, D, N, powers, X), VT));
   out.push_back(f.mul(VT, viscosityIVT(f, species.molar_mass, sc.viscosity, D, powers, X, lean)));
-  auto d = density_diffusivity(f, species.molar_mass, sc.diffusivity, D, mean_molar_mass_VTN, VT, powers, X);
+  auto d = density_diffusivity(f, species.molar_mass, sc.diffusivity, D, mean_molar_mass_VTN, VT, powers, X, lean);
   out.insert(out.end(), d.begin(), d.end());
   return out;
 }
