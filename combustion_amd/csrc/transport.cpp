@@ -333,23 +333,53 @@ std::vector<E> density_diffusivity(Block& f, const std::vector<double>& W, const
                                    const std::vector<E>& lnT, const std::vector<E>& x, bool lean = false) {
   const size_t K = x.size();
   std::vector<std::optional<E>> S(K);
+  auto accumulate = [&](size_t k, size_t j, E Rkj) {
+    E t = f.mul(x[j], Rkj);
+    S[k] = f.defv(S[k] ? f.add(*S[k], t) : t, "S" + std::to_string(k) + "_" + std::to_string(j));
+    t = f.mul(x[k], Rkj);
+    S[j] = f.defv(S[j] ? f.add(*S[j], t) : t, "S" + std::to_string(j) + "_" + std::to_string(k));
+  };
+  // Lean form: 1/(c0 + c1 ℓ + …) as (1/c0)/y with y = 1 + (c1/c0) ℓ + … (about 1 over the fits' range), and the
+  // reciprocals of kBatch consecutive pairs' y from one reciprocal of their product (prefix products, then
+  // back substitution: 3 multiplications per pair instead of a v_rcp_f64 and its Newton step).
+  constexpr size_t kBatch = 4;
+  struct Pending {
+    size_t k, j;
+    E y;
+    double scale;
+  };
+  std::vector<Pending> batch;
+  auto flush = [&]() {
+    if (batch.empty()) return;
+    const size_t n = batch.size();
+    std::vector<E> prefix = {batch[0].y};
+    for (size_t i = 1; i < n; i++) prefix.push_back(f.mul(prefix[i - 1], batch[i].y));
+    E inv = f.div(f.c(1.), prefix[n - 1]);
+    std::vector<E> R(n);
+    for (size_t i = n - 1; i > 0; i--) {
+      R[i] = f.mul(inv, prefix[i - 1]);
+      inv = f.mul(inv, batch[i].y);
+    }
+    R[0] = inv;
+    for (size_t i = 0; i < n; i++)
+      accumulate(batch[i].k, batch[i].j,
+                 f.defv(f.mul(f.c(batch[i].scale), R[i]), "R" + std::to_string(batch[i].k) + "_" + std::to_string(batch[i].j)));
+    batch.clear();
+  };
   for (size_t k = 0; k < K; k++)
     for (size_t j = 0; j < k; j++) {
       const double* c = &P[(k * K + j) * D];
-      E Rkj;
       if (lean && D >= 2 && c[0] != 0.) {
-        // 1/(c0 + c1 ℓ + …) as (1/c0)/(1 + (c1/c0) ℓ + …)
         std::vector<E> terms = {f.c(1.)};
         for (int m = 1; m < D; m++) terms.push_back(f.mul(f.c(c[m] / c[0]), lnT[m]));
-        Rkj = f.defv(f.div(f.c(1. / c[0]), f.sum(terms)), "R" + std::to_string(k) + "_" + std::to_string(j));
+        batch.push_back({k, j, f.sum(terms), 1. / c[0]});
+        if (batch.size() == kBatch) flush();
       } else {
-        Rkj = f.defv(f.div(f.c(1.), dot_consts(f, c, lnT)), "R" + std::to_string(k) + "_" + std::to_string(j));
+        flush();
+        accumulate(k, j, f.defv(f.div(f.c(1.), dot_consts(f, c, lnT)), "R" + std::to_string(k) + "_" + std::to_string(j)));
       }
-      E t = f.mul(x[j], Rkj);
-      S[k] = f.defv(S[k] ? f.add(*S[k], t) : t, "S" + std::to_string(k) + "_" + std::to_string(j));
-      t = f.mul(x[k], Rkj);
-      S[j] = f.defv(S[j] ? f.add(*S[j], t) : t, "S" + std::to_string(j) + "_" + std::to_string(k));
     }
+  flush();
   std::vector<E> out;
   for (size_t k = 0; k < K; k++) out.push_back(f.div(f.sub(mean_molar_mass_VTN, f.mul(f.mul(VT, f.c(W[k])), x[k])), *S[k]));
   return out;
