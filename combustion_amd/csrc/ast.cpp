@@ -243,4 +243,38 @@ OpCounts count(const Linear& l) {
   return c;
 }
 
+int32_t peak_live(const Linear& l) {
+  std::vector<int64_t> last(l.n_registers, -1);
+  std::vector<char> constant(l.n_registers, 0);
+  const int64_t end = (int64_t)l.code.size();
+  for (int64_t t = 0; t < end; t++) {
+    const Instruction& i = l.code[t];
+    if (i.op == LOp::Const) constant[i.dst] = 1;
+    const bool unary = i.op == LOp::Neg || i.op == LOp::Sqrt || i.op == LOp::Exp || i.op == LOp::Ln || i.op == LOp::Sq;
+    if (i.op != LOp::Const) last[i.a] = t;
+    if (i.op != LOp::Const && !unary) last[i.b] = t;
+    if (i.op == LOp::Select) last[i.c] = t;
+  }
+  for (int32_t r : l.output) last[r] = end;
+  // +1 at a definition (inputs at -1), -1 after the last use
+  std::vector<int32_t> delta(end + 2, 0);
+  for (int32_t r = 0; r < l.n_registers; r++) {
+    if (constant[r] || last[r] < 0) continue;
+    delta[0] += 1;  // placeholder, corrected below for instruction results
+    delta[last[r] + 1] -= 1;
+  }
+  for (int64_t t = 0; t < end; t++) {
+    const Instruction& i = l.code[t];
+    if (constant[i.dst] || last[i.dst] < 0) continue;
+    delta[0] -= 1;
+    delta[t] += 1;
+  }
+  int32_t live = 0, peak = 0;
+  for (int64_t t = 0; t <= end; t++) {
+    live += delta[t];
+    peak = std::max(peak, live);
+  }
+  return peak;
+}
+
 }  // namespace combustion::ast

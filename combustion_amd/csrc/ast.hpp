@@ -111,4 +111,9 @@ struct OpCounts {
 };
 OpCounts count(const Linear& l);
 
+// Peak number of values live at once in program order (inputs and instruction results from definition to last
+// use; constants excluded, since a compiler rematerializes them): the register demand of a one-lane-per-state
+// kernel before any rescheduling.
+int32_t peak_live(const Linear& l);
+
 }  // namespace combustion::ast

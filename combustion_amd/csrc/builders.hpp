@@ -75,5 +75,8 @@ struct ChemistryOptions {
   bool lean_transport = false;
 };
 ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
+// The registers of a linearized rates program that hold the clamped exp(-G_k/RT) (negative) or exp(G_k/RT) values
+// (reaction.rs:236-268), in order of definition.
+std::vector<int32_t> gibbs_exponentials(const ast::Linear& l, bool negative);
 
 }  // namespace combustion

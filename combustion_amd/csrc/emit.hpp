@@ -1,6 +1,7 @@
 // Emitters: per-state programs and mechanisms to HIP source for gfx950.
 #pragma once
 #include <string>
+#include <vector>
 
 #include "ast.hpp"
 
@@ -22,6 +23,13 @@ struct StraightOptions {
   // to [-746, 710] (outside it e^x rounds to 0 or overflows to +inf), Cody-Waite reduction to |r| <= ln(2)/2 with
   // n taken from the low word of x log2(e) + 1.5·2^52, a degree-11 polynomial and v_ldexp_f64: no range branches.
   bool exp_poly = false;
+  // Registers kept in LDS instead of VGPRs (one FP64 slot per register and thread, [slot][thread] so that a wave's
+  // accesses are consecutive): stored where defined and read where used. Every `fence_every` instructions a
+  // compiler-only memory fence (no instruction) ends the segment, so that the compiler reloads such a value in each
+  // segment that uses it instead of keeping the first load live across the program. For one-lane-per-state
+  // kernels whose live values exceed the register file: LDS holds what would otherwise spill to scratch.
+  std::vector<int32_t> lds_registers;
+  unsigned fence_every = 128;
 };
 // Straight-line kernel: signature (const double* const* in, double* const* out, unsigned n, unsigned offset).
 std::string emit_straight(const ast::Linear& l, const std::string& name, const StraightOptions& o);

@@ -75,15 +75,34 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
     << "(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n, unsigned offset) {\n"
     << "  const unsigned id = offset + blockIdx.x * blockDim.x + threadIdx.x;\n"
     << "  if (id >= n) return;\n";
+  // LDS-resident registers: slot of each, -1 for the others.
+  std::vector<int32_t> slot(l.n_registers, -1);
+  int32_t n_slots = 0;
+  for (int32_t r : o.lds_registers) {
+    if (r < l.n_input || r >= l.n_registers) throw ast::Error("emit: an LDS register must be an instruction result");
+    if (slot[r] < 0) slot[r] = n_slots++;
+  }
+  if (n_slots) {
+    if ((size_t)n_slots * o.block_size * 8 > 160 * 1024) throw ast::Error("emit: LDS-resident registers exceed 160 KB per block");
+    s << "  __shared__ double lds_v[" << n_slots * o.block_size << "];\n"
+      << "  const unsigned tid = threadIdx.x;\n";
+  }
+  auto operand = [&](int32_t r) {
+    return slot[r] >= 0 ? "lds_v[" + std::to_string(slot[r] * o.block_size) + " + tid]" : "r" + std::to_string(r);
+  };
   for (int i = 0; i < l.n_input; i++) s << "  const " << F << " r" << i << " = (" << F << ")in[" << i << "][id];\n";
   std::vector<bool> is_bool(l.n_registers, false);
   // Outputs are stored as soon as their register is computed, ending their live range early.
   std::vector<std::vector<size_t>> stores(l.n_registers);
   for (size_t o = 0; o < l.output.size(); o++)
     if (l.output[o] >= l.n_input) stores[l.output[o]].push_back(o);
+  size_t emitted = 0;
   for (const ast::Instruction& i : l.code) {
-    const std::string d = "r" + std::to_string(i.dst), a = "r" + std::to_string(i.a), b = "r" + std::to_string(i.b),
-                      c = "r" + std::to_string(i.c);
+    if (n_slots && o.fence_every && ++emitted % o.fence_every == 0) s << "  asm volatile(\"\" ::: \"memory\");\n";
+    const bool unary = i.op == LOp::Const;
+    const std::string d = "r" + std::to_string(i.dst), a = unary ? "" : operand(i.a),
+                      b = i.op == LOp::Neg || i.op == LOp::Sqrt || i.op == LOp::Exp || i.op == LOp::Ln || i.op == LOp::Sq || unary ? "" : operand(i.b),
+                      c = i.op == LOp::Select ? operand(i.c) : "";
     if (i.op == LOp::LessOrEqual) {
       is_bool[i.dst] = true;
       s << "  const bool " << d << " = " << a << " <= " << b << ";\n";
@@ -114,6 +133,7 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
       default: break;
     }
     s << ";\n";
+    if (slot[i.dst] >= 0) s << "  " << operand(i.dst) << " = " << d << ";\n";
     for (size_t o : stores[i.dst]) s << "  out[" << o << "][id] = (double)" << d << ";\n";
   }
   for (size_t o = 0; o < l.output.size(); o++)

@@ -383,6 +383,17 @@ std::vector<std::string> names(std::initializer_list<const char*> head, size_t n
 
 }  // namespace
 
+std::vector<int32_t> gibbs_exponentials(const ast::Linear& l, bool negative) {
+  const double bound = negative ? exp_neg_clamp : exp_clamp;
+  std::vector<char> clamp(l.n_registers, 0);
+  std::vector<int32_t> out;
+  for (const ast::Instruction& i : l.code) {
+    if (i.op == ast::LOp::Const && i.k == bound) clamp[i.dst] = 1;
+    if (i.op == ast::LOp::Min && (clamp[i.a] || clamp[i.b])) out.push_back(i.dst);
+  }
+  return out;
+}
+
 std::vector<E> species_rates(Block& f, const Mechanism& m, const Temperatures& T, E concentration, const std::vector<E>& C) {
   std::vector<std::optional<E>> sums(m.active);
   Reactions cR(f, m, T, concentration, C);

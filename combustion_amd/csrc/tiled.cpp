@@ -335,7 +335,14 @@ std::string rates_straight_text(const ProgramView& p, bool fast) {
   so.newton_steps = 1;
   so.fast_transcendentals = fast;
   so.exp_poly = true;
-  return emit_straight(ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o)), fast ? "rates_fast" : "rates_fp64", so);
+  const ast::Linear l = ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o));
+  // A program that needs more than the 128 doubles of two waves per SIMD runs at one wave with its values in VGPRs,
+  // AGPRs and scratch; its clamped exp(-G_k/RT) values then live in LDS (K_a slots of 2 KB per block of 256): at K=53
+  // (291 live doubles) that cuts the kernel's scratch from 948 to 348 B per lane and its time by 6 %; also keeping 26
+  // of the exp(+G_k/RT) values there (the rest of the 160 KB) was not faster. Beyond 80 slots they stay in registers.
+  if (ast::peak_live(l) > 128) so.lds_registers = gibbs_exponentials(l, true);
+  if (so.lds_registers.size() * so.block_size * 8 > 160 * 1024) so.lds_registers.clear();
+  return emit_straight(l, fast ? "rates_fast" : "rates_fp64", so);
 }
 
 // rates_mixed with its tables in device memory; writes ω to out[0..Ka).
