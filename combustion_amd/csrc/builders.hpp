@@ -13,9 +13,9 @@ struct Temperatures {
   ast::E lnT, T, T2, T3, T4, rcpT, rcpT2;
 };
 
-// reaction.rs:312-327: net production rate of every active species.
+// reaction.rs:312-327: net production rate of every active species (lean: ChemistryOptions::lean_rates).
 std::vector<ast::E> species_rates(ast::Block& f, const Mechanism& m, const Temperatures& Ts, ast::E concentration,
-                                  const std::vector<ast::E>& concentrations);
+                                  const std::vector<ast::E>& concentrations, bool lean = false);
 // reaction.rs:356-391: inputs [P/R, R/P, T, V, n_0..n_{K-2}], outputs [T dT/dt / T * T, dV/dt, ω_0..ω_{Ka-1}].
 ast::Function rates(const Mechanism& m);
 // reaction.rs:329-354: inputs [P/R, T, V, n_0..n_{K-2}], outputs the net rate of progress of every reaction.
@@ -73,6 +73,10 @@ struct ChemistryOptions {
   // two VGPR moves): Wilke's (a_kj + b_kj v_k/v_j)² as a_kj² (1 + ṽ_k/ṽ_j)² with ṽ_k = v_k W_k^(-1/4), and the
   // binary-diffusion fit c0 + c1 ℓ + … as c0 (1 + (c1/c0) ℓ + …).
   bool lean_transport = false;
+  // Rates program shaped for the one-lane-per-state kernels: each reaction's products start from its own rate
+  // constant, q = k ΠC^ν' − k (1/K) ΠC^ν'', so that no product of concentrations or of exp(±G/RT) factors is shared
+  // between reactions (a shared one stays live from its first use to its last); the same values to rounding.
+  bool lean_rates = false;
 };
 ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
 // The registers of a linearized rates program that hold the clamped exp(-G_k/RT) (negative) or exp(G_k/RT) values

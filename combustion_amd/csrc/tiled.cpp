@@ -330,6 +330,7 @@ void check_program(const ProgramView& p) {
 std::string rates_straight_text(const ProgramView& p, bool fast) {
   ChemistryOptions o;
   o.transport = false;
+  o.lean_rates = true;
   StraightOptions so;
   so.reciprocal = true;
   so.newton_steps = 1;
@@ -337,9 +338,11 @@ std::string rates_straight_text(const ProgramView& p, bool fast) {
   so.exp_poly = true;
   const ast::Linear l = ast::linearize(chemistry(*p.mechanism, nullptr, p.chemistry->scales, o));
   // A program that needs more than the 128 doubles of two waves per SIMD runs at one wave with its values in VGPRs,
-  // AGPRs and scratch; its clamped exp(-G_k/RT) values then live in LDS (K_a slots of 2 KB per block of 256): at K=53
-  // (291 live doubles) that cuts the kernel's scratch from 948 to 348 B per lane and its time by 6 %; also keeping 26
-  // of the exp(+G_k/RT) values there (the rest of the 160 KB) was not faster. Beyond 80 slots they stay in registers.
+  // AGPRs and scratch; its clamped exp(-G_k/RT) values then live in LDS (K_a slots of 2 KB per block of 256). At
+  // K=53 (about 230 live doubles with the lean rates form) the kernel then needs no scratch and takes 1.25 ms per 2^20
+  // states against 1.30 ms with the values in registers; keeping some exp(+G_k/RT) values there too, a fence every
+  // 64 or 256 instructions, or two waves per SIMD with 40 slots (1.6 ms, 1.2 KB of scratch) were not faster.
+  // Beyond 80 slots the values stay in registers.
   if (ast::peak_live(l) > 128) so.lds_registers = gibbs_exponentials(l, true);
   if (so.lds_registers.size() * so.block_size * 8 > 160 * 1024) so.lds_registers.clear();
   return emit_straight(l, fast ? "rates_fast" : "rates_fp64", so);
