@@ -267,11 +267,12 @@ int cb_build_transport(const cb_mechanism* m, int32_t D, double T0, double visco
 int cb_build_chemistry(const cb_mechanism* m, int32_t flags, int32_t D, double T0, double viscosity, double conductivity, cb_program** out) {
   return guard([&] {
     require(m && out, "null argument");
-    require(flags > 0 && flags < 8, "flags must combine CB_RATES, CB_TRANSPORT, CB_RHS");
+    require((flags & 7) > 0 && flags < 16, "flags must combine CB_RATES, CB_TRANSPORT, CB_RHS (and CB_LEAN)");
     ChemistryDescription d;
     d.options.rates = flags & CB_RATES;
     d.options.transport = flags & CB_TRANSPORT;
     d.options.rhs = flags & CB_RHS;
+    d.options.lean_rates = d.options.lean_transport = flags & CB_LEAN;
     d.scales = transport_scales(*m->m, T0);
     if (viscosity > 0) d.scales.viscosity = viscosity;
     if (conductivity > 0) d.scales.conductivity = conductivity;

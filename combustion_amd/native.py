@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcombustion_amd.so")
 
-RATES, TRANSPORT, RHS = 1, 2, 4
+RATES, TRANSPORT, RHS, LEAN = 1, 2, 4, 8
 FORM_STRAIGHT, FORM_FP64, FORM_MIXED, FORM_FAST = 0, 1, 2, 3
 FORMS = {"straight": FORM_STRAIGHT, "fp64": FORM_FP64, "mixed": FORM_MIXED, "fast": FORM_FAST}
 

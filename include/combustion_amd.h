@@ -45,8 +45,12 @@ enum cb_form {
   CB_FORM_FAST = 3      /* FP64 straight-line rates with FP32 cores for exp and ln, FP32 transport */
 };
 
-/* Sections of the per-state program built by cb_build_chemistry. */
-enum cb_chemistry_flags { CB_RATES = 1, CB_TRANSPORT = 2, CB_RHS = 4 };
+/* Sections of the per-state program built by cb_build_chemistry. CB_LEAN selects the shapes that the
+ * one-lane-per-state FP64 kernels compile (the same values to a few ulps per operation): rate-of-progress products
+ * started from each reaction's rate constant, one-constant Wilke and binary-diffusion pair terms, and diffusion
+ * reciprocals four pairs at a time. No reference counterpart: it exists so that the interpreter can check those
+ * shapes against the oracle. */
+enum cb_chemistry_flags { CB_RATES = 1, CB_TRANSPORT = 2, CB_RHS = 4, CB_LEAN = 8 };
 
 const char* cb_last_error(void);
 const char* cb_version(void);
