@@ -77,6 +77,8 @@ struct ChemistryOptions {
   // constant, q = k ΠC^ν' − k (1/K) ΠC^ν'', so that no product of concentrations or of exp(±G/RT) factors is shared
   // between reactions (a shared one stays live from its first use to its last); the same values to rounding.
   bool lean_rates = false;
+  // Transport program for FP32 evaluation: the diffusion sums S_k kept at or above FLT_MIN (see density_diffusivity).
+  bool f32_range = false;
 };
 ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
 // The registers of a linearized rates program that hold the clamped exp(-G_k/RT) (negative) or exp(G_k/RT) values

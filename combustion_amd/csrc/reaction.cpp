@@ -519,7 +519,7 @@ ast::Function reactions_rates(const Mechanism& m) {
 
 // Transport section of the per-state program (transport.cpp).
 std::vector<E> transport_outputs(Block& f, const Species& s, const Polynomials& p, const TransportScales& scales, E N, E theta, bool lean,
-                                 const std::vector<E>& x, E& mean_molar_massN);
+                                 bool f32_range, const std::vector<E>& x, E& mean_molar_massN);
 
 ast::Function chemistry(const Mechanism& m, const Polynomials* poly, const TransportScales& scales, const ChemistryOptions& o) {
   const size_t K = m.species.len();
@@ -543,7 +543,7 @@ ast::Function chemistry(const Mechanism& m, const Polynomials* poly, const Trans
   if (o.transport) {
     E theta = f.defv(f.mul(T, f.c(1. / scales.T0)), "T/T0");
     E W_N;
-    std::vector<E> tr = transport_outputs(f, m.species, *poly, scales, p.concentration, theta, o.lean_transport,
+    std::vector<E> tr = transport_outputs(f, m.species, *poly, scales, p.concentration, theta, o.lean_transport, o.f32_range,
                                           std::vector<E>(p.C.begin(), p.C.end() - 1), W_N);
     out.insert(out.end(), tr.begin(), tr.end());
   }

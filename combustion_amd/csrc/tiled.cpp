@@ -418,6 +418,7 @@ std::string transport_straight_text(const ProgramView& p, bool f64) {
   ChemistryOptions o;
   o.rates = false;
   o.lean_transport = true;
+  o.f32_range = !f64;
   StraightOptions so;
   so.reciprocal = f64;
   so.newton_steps = 1;

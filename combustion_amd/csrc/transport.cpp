@@ -1,5 +1,6 @@
 // Transport: kinetic-theory species properties, their polynomial fits, and the mixture-averaged program.
 // Restates transport.rs:14-214 (and its collision-integral tables, transport/collision_integrals.rs:1-82).
+#include <cfloat>
 #include <cmath>
 #include <stdexcept>
 
@@ -330,7 +331,7 @@ E viscosityIVT(Block& f, const std::vector<double>& W, const std::vector<double>
 
 // transport.rs:162-177
 std::vector<E> density_diffusivity(Block& f, const std::vector<double>& W, const std::vector<double>& P, int D, E mean_molar_mass_VTN, E VT,
-                                   const std::vector<E>& lnT, const std::vector<E>& x, bool lean = false) {
+                                   const std::vector<E>& lnT, const std::vector<E>& x, bool lean = false, bool f32_range = false) {
   const size_t K = x.size();
   std::vector<std::optional<E>> S(K);
   auto accumulate = [&](size_t k, size_t j, E Rkj) {
@@ -381,7 +382,18 @@ std::vector<E> density_diffusivity(Block& f, const std::vector<double>& W, const
     }
   flush();
   std::vector<E> out;
-  for (size_t k = 0; k < K; k++) out.push_back(f.div(f.sub(mean_molar_mass_VTN, f.mul(f.mul(VT, f.c(W[k])), x[k])), *S[k]));
+  for (size_t k = 0; k < K; k++) {
+    // FP32 evaluation: S_k of a species whose partners are all traces below ~1e-38 underflows to 0 (x/0 = ±inf);
+    // FLT_MIN instead keeps D_k finite, within the error floor 1/S_k of such a state. A negative S_k (fits
+    // extrapolated beyond their temperature range) is kept as it is.
+    E s = *S[k];
+    if (f32_range) {
+      const int32_t id = f.value("S_k in FP32 range");
+      f.select(f.le(f.c(0.), s), {f.max(s, f.c((double)FLT_MIN))}, {s}, {id});
+      s = f.v(id);
+    }
+    out.push_back(f.div(f.sub(mean_molar_mass_VTN, f.mul(f.mul(VT, f.c(W[k])), x[k])), s));
+  }
   return out;
 }
 
@@ -399,7 +411,7 @@ Scaled scale(const Polynomials& p, double T0, double viscosity, double conductiv
 
 // transport.rs:185-209 from the mixture inputs already in scope; returns [λ', μ', ρD'_k...] before the caller's scaling.
 std::vector<E> properties_body(Block& f, const Species& species, const Polynomials& p, const Scaled& sc, E N, E theta,
-                               const std::vector<E>& nonbulk, E& mean_molar_massN, bool lean = false) {
+                               const std::vector<E>& nonbulk, E& mean_molar_massN, bool lean = false, bool f32_range = false) {
   const size_t K = species.len();
   const int D = p.D;
   E lnT = f.defv(f.ln(theta), "lnT");
@@ -419,7 +431,7 @@ std::vector<E> properties_body(Block& f, const Species& species, const Polynomia
   std::vector<E> out;
   out.push_back(f.mul(conductivityNIVT(f, sc.conductivity, D, N, powers, X), VT));
   out.push_back(f.mul(VT, viscosityIVT(f, species.molar_mass, sc.viscosity, D, powers, X, lean)));
-  auto d = density_diffusivity(f, species.molar_mass, sc.diffusivity, D, mean_molar_mass_VTN, VT, powers, X, lean);
+  auto d = density_diffusivity(f, species.molar_mass, sc.diffusivity, D, mean_molar_mass_VTN, VT, powers, X, lean, f32_range);
   out.insert(out.end(), d.begin(), d.end());
   return out;
 }
@@ -474,8 +486,9 @@ ast::Function transport_device_function(const Species& species, const Polynomial
 
 // Transport section of the per-state program: physical λ, μ and D_k = ρD_k/ρ with ρ = Σ_k W_k C_k.
 std::vector<E> transport_outputs(Block& f, const Species& s, const Polynomials& p, const TransportScales& scales, E N, E theta, bool lean,
-                                 const std::vector<E>& x, E& mean_molar_massN) {
-  auto out = properties_body(f, s, p, scale(p, scales.T0, scales.viscosity, scales.conductivity), N, theta, x, mean_molar_massN, lean);
+                                 bool f32_range, const std::vector<E>& x, E& mean_molar_massN) {
+  auto out = properties_body(f, s, p, scale(p, scales.T0, scales.viscosity, scales.conductivity), N, theta, x, mean_molar_massN, lean,
+                             f32_range);
   out[0] = f.mul(f.c(scales.conductivity), out[0]);
   out[1] = f.mul(f.c(scales.viscosity), out[1]);
   E scale_D = f.defv(f.div(f.c(scales.viscosity), mean_molar_massN), "η0/ρ");
