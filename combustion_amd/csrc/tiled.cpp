@@ -469,12 +469,37 @@ class Tiled : public SoAForm {
     }
   }
 
+  ~Tiled() override {
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
+    if (side_) (void)hipStreamDestroy(side_);
+  }
+
  protected:
+  // The rates and transport kernels read the same inputs and write disjoint outputs: the transport kernel runs on a
+  // second stream of the module's (default priority), forked from and joined back into the caller's stream, so that
+  // the two kernels' blocks share the CUs at the ends of each launch (2.27 against 2.32 ms per 2^20 states at K=53).
   void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
+    const bool both = (mixed_ || fp64_) && (straight_ || transport_);
+    hipStream_t ts = stream;
+    if (both) {
+      if (!side_) {
+        device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        device::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
+        device::check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreateWithFlags");
+      }
+      device::check(hipEventRecord(fork_, stream), "hipEventRecord");
+      device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
+      ts = side_;
+    }
     if (mixed_) mixed_->launch(in, out, n, stream);
     if (fp64_) fp64_->launch(in, out, n, stream);
-    if (straight_) straight_->launch(in, out + n_rates_, n, stream);
-    if (transport_) transport_->launch(in, out + n_rates_, n, stream);
+    if (straight_) straight_->launch(in, out + n_rates_, n, ts);
+    if (transport_) transport_->launch(in, out + n_rates_, n, ts);
+    if (both) {
+      device::check(hipEventRecord(join_, side_), "hipEventRecord");
+      device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
+    }
   }
 
  private:
@@ -483,6 +508,8 @@ class Tiled : public SoAForm {
   std::unique_ptr<TransportStraight> straight_;
   std::unique_ptr<TransportTiled> transport_;
   int n_rates_ = 0;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
 };
 
 std::vector<uint16_t> to_bf16(const std::vector<float>& v) {
