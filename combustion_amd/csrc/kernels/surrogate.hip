@@ -275,3 +275,8 @@ surrogate_transport(const double* const* __restrict__ in, double* const* __restr
     for (int o = wave; o < NF; o += 4) out[o][s] = (double)L.y[o][lane];
   if (wave == 0 && valid && !in_box) fallback[atomicAdd(fallback_count, 1u)] = s;
 }
+
+// Clears the fallback counter before a launch: a kernel rather than a memset, so that a captured graph replays it.
+extern "C" __global__ void surrogate_clear(unsigned* fallback_count) {
+  if (threadIdx.x == 0) fallback_count[threadIdx.x] = 0u;
+}

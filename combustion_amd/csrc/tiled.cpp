@@ -592,6 +592,7 @@ class Surrogate : public SoAForm {
       offsets.push_back(L.b);
     }
     kernel_.load(device::compile(surrogate_text(K, lay, DK, NM), "surrogate_transport", hip_options(true)), "surrogate_transport");
+    device::check(hipModuleGetFunction(&clear_, kernel_.module, "surrogate_clear"), "hipModuleGetFunction surrogate_clear");
     d_w16_ = upload(to_bf16(w.params));
     d_params_ = upload(w.params);
     d_offsets_ = upload(offsets);
@@ -617,6 +618,9 @@ class Surrogate : public SoAForm {
                       (void*)d_hi_, (void*)d_o_mean_, (void*)d_o_std_, (void*)d_bc_, (void*)d_bv_, (void*)d_bd_, (void*)d_rw_, (void*)d_count_,
                       (void*)d_fallback_, (void*)d_major_, (void*)d_slot_, (void*)d_wa_, (void*)d_wb_})
       if (ptr) (void)hipFree(ptr);
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
+    if (side_) (void)hipStreamDestroy(side_);
   }
   // States the last call sent to the plain transport kernel (waits for the stream).
   unsigned fallback_count(hipStream_t stream) {
@@ -634,18 +638,36 @@ class Surrogate : public SoAForm {
       device::check(hipMalloc(&d_fallback_, n * sizeof(unsigned)), "hipMalloc");
       capacity_ = n;
     }
-    if (rates_) rates_->launch(in, out, n, stream);
+    // The rates kernel on the caller's stream; the surrogate and its fallback on a side stream forked from and joined
+    // back into it (the two read the same inputs and write disjoint outputs), as in the tiled forms.
+    hipStream_t ts = stream;
+    if (rates_) {
+      if (!side_) {
+        device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        device::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
+        device::check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreateWithFlags");
+      }
+      device::check(hipEventRecord(fork_, stream), "hipEventRecord");
+      device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
+      ts = side_;
+      rates_->launch(in, out, n, stream);
+    }
     double* const* tout = out + n_rates_;
-    device::check(hipMemsetAsync(d_count_, 0, sizeof(unsigned), stream), "hipMemsetAsync");
+    void* clear_args[] = {&d_count_};
+    device::check(hipModuleLaunchKernel(clear_, 1, 1, 1, 64, 1, 1, 0, ts, clear_args, nullptr), "launch surrogate_clear");
     unsigned n32 = (unsigned)n;
     void* args[] = {&in, &tout, &n32, &d_w16_, &d_params_, &d_offsets_, &d_f_mean_, &d_f_rstd_, &d_lo_, &d_hi_,
                     &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_major_, &d_slot_, &d_wa_, &d_wb_, &d_bd_, &d_rw_, &T0_,
                     &d_fallback_, &d_count_};
     const unsigned per_block = 64;
-    device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + per_block - 1) / per_block), 1, 1, 256, 1, 1, 0, stream, args,
+    device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + per_block - 1) / per_block), 1, 1, 256, 1, 1, 0, ts, args,
                                         nullptr),
                   "launch surrogate_transport");
-    transport_->launch(in, tout, n, stream, d_fallback_, d_count_);
+    transport_->launch(in, tout, n, ts, d_fallback_, d_count_);
+    if (rates_) {
+      device::check(hipEventRecord(join_, side_), "hipEventRecord");
+      device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
+    }
   }
 
  private:
@@ -661,6 +683,9 @@ class Surrogate : public SoAForm {
   float T0_ = 1000.f;
   unsigned *d_count_ = nullptr, *d_fallback_ = nullptr;
   size_t capacity_ = 0;
+  hipFunction_t clear_ = nullptr;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
 };
 
 }  // namespace
