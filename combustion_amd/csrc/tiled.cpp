@@ -423,9 +423,9 @@ std::string transport_straight_text(const ProgramView& p, bool f64) {
   so.reciprocal = f64;
   so.newton_steps = 1;
   so.f32 = !f64;
-  // FP64: the lean viscosity program fits two waves per SIMD (255 VGPRs, no AGPRs), 30 % faster at K=53 than one
-  // wave with 329 registers; the FP32 kernel already holds two waves (251 VGPRs) and a third one was slower.
-  if (f64) so.min_waves = 2;
+  // FP64: the lean transport program fits two waves per SIMD (243 VGPRs, no AGPRs). FP32: three waves (168 VGPRs,
+  // one spilled register) beat two (246 VGPRs) by 1.5 % on the fast form's step at K=53.
+  so.min_waves = f64 ? 2 : 3;
   return emit_straight(ast::linearize(chemistry(*p.mechanism, p.chemistry->polynomials.get(), p.chemistry->scales, o)),
                        f64 ? "transport_fp64" : "transport_f32", so);
 }
