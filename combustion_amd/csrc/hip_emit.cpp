@@ -99,9 +99,11 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   size_t emitted = 0;
   for (const ast::Instruction& i : l.code) {
     if (n_slots && o.fence_every && ++emitted % o.fence_every == 0) s << "  asm volatile(\"\" ::: \"memory\");\n";
-    const bool unary = i.op == LOp::Const;
-    const std::string d = "r" + std::to_string(i.dst), a = unary ? "" : operand(i.a),
-                      b = i.op == LOp::Neg || i.op == LOp::Sqrt || i.op == LOp::Exp || i.op == LOp::Ln || i.op == LOp::Sq || unary ? "" : operand(i.b),
+    // Operands by arity: a constant has none, Neg/Sqrt/Exp/Ln/Sq one, Select three, the rest two. An LDS-resident
+    // operand reads its slot (the thread index is below the block size, which __launch_bounds__ enforces).
+    const bool none = i.op == LOp::Const;
+    const bool one = i.op == LOp::Neg || i.op == LOp::Sqrt || i.op == LOp::Exp || i.op == LOp::Ln || i.op == LOp::Sq;
+    const std::string d = "r" + std::to_string(i.dst), a = none ? "" : operand(i.a), b = none || one ? "" : operand(i.b),
                       c = i.op == LOp::Select ? operand(i.c) : "";
     if (i.op == LOp::LessOrEqual) {
       is_bool[i.dst] = true;
