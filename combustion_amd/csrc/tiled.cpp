@@ -492,6 +492,7 @@ class Tiled : public SoAForm {
       device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
       ts = side_;
     }
+    // The longer rates kernel goes first (the other order measured 2.29 against 2.25 ms).
     if (mixed_) mixed_->launch(in, out, n, stream);
     if (fp64_) fp64_->launch(in, out, n, stream);
     if (straight_) straight_->launch(in, out + n_rates_, n, ts);
