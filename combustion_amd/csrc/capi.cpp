@@ -10,6 +10,8 @@
 #include "builders.hpp"
 #include "device.hpp"
 #include "emit.hpp"
+#include "export.hpp"
+#include "ignition.hpp"
 #include "forms.hpp"
 #include "mechanism.hpp"
 #include "mlp.hpp"
@@ -41,11 +43,20 @@ namespace {
 
 thread_local std::string last_error;
 
+// An error already reported by a nested entry point: its status and message pass through unchanged.
+struct StatusError : std::runtime_error {
+  int status;
+  StatusError(int s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
 template <class F>
 int guard(F&& f) {
   try {
     f();
     return CB_OK;
+  } catch (const StatusError& e) {
+    last_error = e.what();
+    return e.status;
   } catch (const yaml::Error& e) {
     last_error = std::string("parse: ") + e.what();
     return CB_ERR_PARSE;
@@ -207,6 +218,28 @@ int cb_mechanism_json(const cb_mechanism* m, char* buffer, size_t capacity, size
   return write_text(mechanism_json(*m->m), buffer, capacity, needed);
 }
 
+int cb_ignition_state_parse(const cb_mechanism* m, const char* text, double* temperature, double* pressure, double* amounts) {
+  return guard([&] {
+    require(m && text && temperature && pressure && amounts, "null argument");
+    const IgnitionState st = ignition_state(*m->m, text);
+    *temperature = st.temperature;
+    *pressure = st.pressure;
+    std::copy(st.amounts.begin(), st.amounts.end(), amounts);
+  });
+}
+
+int cb_ignition_state_load(const cb_mechanism* m, const char* path, double* temperature, double* pressure, double* amounts) {
+  return guard([&] {
+    require(m && path, "null argument");
+    std::ifstream f(path);
+    if (!f) throw ArgumentError(std::string("cannot open ") + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const int status = cb_ignition_state_parse(m, ss.str().c_str(), temperature, pressure, amounts);
+    if (status) throw StatusError(status, last_error);
+  });
+}
+
 int cb_transport_fits(const cb_mechanism* m, int32_t D, double T0, double* conductivity, double* viscosity, double* diffusivity) {
   return guard([&] {
     require(m && conductivity && viscosity && diffusivity, "null argument");
@@ -364,26 +397,26 @@ int cb_export_device_function(const cb_mechanism* m, int32_t which, int32_t D, c
   std::string text;
   int status = guard([&] {
     require(m, "null mechanism");
-    require(which >= 0 && which <= 3, "which is CB_EXPORT_RATES, _CONDUCTIVITY, _VISCOSITY or _DIFFUSIVITY");
-    require(D >= 2 && D <= 8, "D out of range");
-    const std::string pre = prefix ? prefix : "";
-    if (which == 0) {
-      text = emit_device_function(ast::linearize(rates_device_function(*m->m)), {"pressure_R", "lnT", "T", "T2", "T3", "T4", "rcpT"},
-                                  "concentrations", pre + "species_rates", DeviceOutput::Out);
-      return;
-    }
-    const TransportScales sc = transport_scales(*m->m);
-    const Polynomials p = fit_polynomials(m->m->species, sc.T0, D);
-    std::vector<std::string> scalars;
-    if (which == 1) scalars.push_back("sum_mole_proportions");
-    if (which == 3) scalars = {"mean_molar_mass_VTN", "VT"};
-    for (int k = 1; k < D; k++) scalars.push_back(k == 1 ? "lnT" : "lnT" + std::to_string(k));
-    const char* names[] = {"", "conductivityNIVT", "viscosityIVT", "density_diffusivity"};
-    const ast::Linear l = ast::linearize(transport_device_function(m->m->species, p, sc.T0, sc.viscosity, sc.conductivity, which));
-    text = emit_device_function(l, scalars, "mole_proportions", pre + names[which], which == 3 ? DeviceOutput::Strided : DeviceOutput::Return);
+    text = export_text(*m->m, which, D, prefix ? prefix : "");
   });
   if (status) return status;
   return write_text(text, buffer, capacity, needed);
+}
+
+int cb_hip_assemble_export(const cb_mechanism* m, int32_t D, cb_module** out) {
+  return guard([&] {
+    require(m, "null mechanism");
+    require(D >= 2 && D <= 8, "D out of range");
+    if (!out) return compile_export(*m->m, D);
+    auto* mod = new cb_module;
+    try {
+      mod->form = assemble_export(*m->m, D);
+    } catch (...) {
+      delete mod;
+      throw;
+    }
+    *out = mod;
+  });
 }
 
 int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size, char* buffer, size_t capacity, size_t* needed) {

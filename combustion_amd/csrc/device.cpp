@@ -94,8 +94,10 @@ Kernel::~Kernel() {
   if (module) (void)hipModuleUnload(module);
 }
 
-void Kernel::load(const std::vector<char>& code, const std::string& name) {
-  check(hipModuleLoadData(&module, code.data()), "hipModuleLoadData");
+void Kernel::load(std::vector<char> code, const std::string& name) {
+  if (module) throw Error("Kernel::load: a module is already loaded");
+  image = std::move(code);
+  check(hipModuleLoadData(&module, image.data()), "hipModuleLoadData");
   check(hipModuleGetFunction(&function, module, name.c_str()), ("hipModuleGetFunction " + name).c_str());
 }
 

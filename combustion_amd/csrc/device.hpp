@@ -21,11 +21,17 @@ std::vector<char> compile(const std::string& source, const std::string& name, co
 std::string cache_directory();
 std::string cache_key(const std::string& source, const std::vector<std::string>& options);
 
+// A loaded code object and one of its kernels. The code object's bytes stay owned by the Kernel for as long as
+// the module is loaded: hipModuleLoadData is not documented to copy the image it is given.
 struct Kernel {
   hipModule_t module = nullptr;
   hipFunction_t function = nullptr;
+  std::vector<char> image;
+  Kernel() = default;
+  Kernel(const Kernel&) = delete;
+  Kernel& operator=(const Kernel&) = delete;
   ~Kernel();
-  void load(const std::vector<char>& code, const std::string& name);
+  void load(std::vector<char> code, const std::string& name);
 };
 
 }  // namespace combustion::device

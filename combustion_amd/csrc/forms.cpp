@@ -53,23 +53,35 @@ StraightOptions straight_options(int form, unsigned block_size) {
 }
 
 // Chemistry programs (without the reactor right-hand side) take the per-section kernels in forms 1-3; form 2 needs
-// them.
-bool use_tiled(const ProgramView& p, int form) { return form == 2 || ((form == 1 || form == 3) && tiled_applies(p)); }
+// them, and so does the tiled-transport modifier.
+bool use_tiled(const ProgramView& p, int form_flags) {
+  const int form = form_flags & kFormBase;
+  if (form_flags & kFormTiledTransport) {
+    if (form < 1 || form > 3) throw ArgumentError("the tiled-transport modifier applies to forms 1-3");
+    if (!tiled_applies(p)) throw Unsupported("the tiled-transport modifier needs a chemistry program without the reactor right-hand side");
+    return true;
+  }
+  return form == 2 || ((form == 1 || form == 3) && tiled_applies(p));
+}
+
+void check_form(int form_flags) {
+  if ((form_flags & ~(kFormBase | kFormTiledTransport)) || (form_flags & kFormBase) > 3) throw ArgumentError("unknown form");
+}
 
 std::string form_source(const ProgramView& p, int form, unsigned block_size) {
-  if (form < 0 || form > 3) throw ArgumentError("unknown form");
+  check_form(form);
   if (use_tiled(p, form)) return tiled_source(p, form, block_size);
   return emit_straight(*p.linear, "program", straight_options(form, block_size));
 }
 
 void compile_form(const ProgramView& p, int form, unsigned block_size) {
-  if (form < 0 || form > 3) throw ArgumentError("unknown form");
+  check_form(form);
   if (use_tiled(p, form)) return compile_tiled(p, form, block_size);
   device::compile(emit_straight(*p.linear, "program", straight_options(form, block_size)), "program", hip_options(form != 0));
 }
 
 std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size) {
-  if (form < 0 || form > 3) throw ArgumentError("unknown form");
+  check_form(form);
   if (use_tiled(p, form)) return assemble_tiled(p, form, block_size);
   return std::make_unique<Straight>(p, straight_options(form, block_size), form != 0);
 }

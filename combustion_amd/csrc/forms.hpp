@@ -48,6 +48,10 @@ class Form {
   virtual void call(const double* const* inputs, size_t n, double* const* outputs, bool device, hipStream_t stream, float* kernel_ms) = 0;
 };
 
+// cb_hip_assemble's form argument: the form in the low bits, modifier flags above (cb_form, cb_form_flags).
+constexpr int kFormBase = 0xF;
+constexpr int kFormTiledTransport = 0x10;
+
 // Common hiprtc options for gfx950 (packed FP32 instructions disabled).
 std::vector<std::string> hip_options(bool contract);
 
@@ -57,7 +61,7 @@ std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned blo
 
 // Tiled forms (tiled.cpp).
 bool tiled_applies(const ProgramView& p);
-// form: 1 FP64, 2 MFMA mixed precision, 3 FP32-core straight-line rates (see cb_form).
+// form: 1 FP64, 2 MFMA mixed precision, 3 FP32-core straight-line rates (see cb_form), with kFormTiledTransport.
 std::string tiled_source(const ProgramView& p, int form, unsigned block_size);
 void compile_tiled(const ProgramView& p, int form, unsigned block_size);
 std::unique_ptr<Form> assemble_tiled(const ProgramView& p, int form, unsigned block_size);

@@ -39,7 +39,7 @@ typedef unsigned short u16;
 
 // Reaction tables (sorted reaction r):
 //   rate[r] = (log2 A, β, Ea/R log2 e, -Σν) of k (Troe: log2 A∞ and the exponent of k∞/A∞);
-//   k0[r] = (log2 A0 (Troe: log2(A0/A∞)), β0, Ea0/R log2 e, 0);  troe[r] = (α, 1/T3, 1/T1 (0: constant α), T2 (0: absent));
+//   k0[r] = (log2 A0 (Troe: log2(A0/A∞)), β0, Ea0/R log2 e, 0);  troe[r] = (α, 1/T3, 1/T1 (0: constant α), T2 (NaN: absent));
 //   slots[r] = (info, 6 ΔG slots as 16-bit byte offsets into the G tables, two per word): info = model |
 //   kinf_const << 4 (model 0 elementary, 1 irreversible, 2 three-body, 3 Lindemann, 4 Troe, 5 padding); a species
 //   with ν = ±2 takes two slots, unused slots point at a zero row;
@@ -138,7 +138,7 @@ __device__ __forceinline__ void half_tile(int h, int rt, int lane, int grp, int 
         const f32x4 t = troe[r];
         const float e3 = exp2f_hw(-my_T * t[1] * (float)LOG2E);
         float Fcent = (1.f - t[0]) * e3 + (t[2] != 0.f ? t[0] * exp2f_hw(-my_T * t[2] * (float)LOG2E) : t[0]);
-        if (t[3] != 0.f) Fcent += exp2f_hw(-t[3] * __builtin_amdgcn_rcpf(my_T) * (float)LOG2E);
+        if (t[3] == t[3]) Fcent += exp2f_hw(-t[3] * __builtin_amdgcn_rcpf(my_T) * (float)LOG2E);  // T2 given (not NaN)
         const float lnFcent = log2f_hw(Fcent) * (float)LN2;
         const float c = -0.4f * LN10F - 0.67f * lnFcent;
         const float N = 0.75f * LN10F - 1.27f * lnFcent;

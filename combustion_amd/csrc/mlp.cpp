@@ -40,8 +40,8 @@ Mlp::Mlp(int n_in, int width, int depth, int n_out) : n_in_(n_in), n_out_(n_out)
   out_pad_ = lay.out_pad;
   n_params_ = lay.n_params;
   for (const auto& L : lay.layers) layers_.push_back({L.in, L.out, L.w, L.b});
-  std::vector<char> code = device::compile(mlp_source, "mlp", hip_options(true));
-  device::check(hipModuleLoadData(&module_, code.data()), "hipModuleLoadData mlp");
+  image_ = device::compile(mlp_source, "mlp", hip_options(true));  // kept while the module is loaded
+  device::check(hipModuleLoadData(&module_, image_.data()), "hipModuleLoadData mlp");
   auto get = [&](hipFunction_t& f, const char* name) { device::check(hipModuleGetFunction(&f, module_, name), name); };
   get(gemm_, "mlp_gemm");
   get(reduce_, "mlp_reduce");

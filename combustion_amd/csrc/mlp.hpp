@@ -61,6 +61,7 @@ class Mlp {
   long in_pad_, out_pad_, n_params_;
   std::vector<Layer> layers_;
   hipModule_t module_ = nullptr;
+  std::vector<char> image_;
   hipFunction_t gemm_, reduce_, colsum_, to_bf16_, mse_, adam_;
 };
 

@@ -1,6 +1,7 @@
-// Launch plumbing shared by every form: SoA pointer tables in device memory, host-buffer staging and
-// kernel timing with HIP events on the caller's stream.
+// Launch plumbing shared by every form: SoA pointer tables in device memory, host-buffer staging, the ordering
+// of a module's calls across streams and kernel timing with HIP events on the caller's stream.
 #pragma once
+#include <map>
 #include <vector>
 
 #include "device.hpp"
@@ -12,9 +13,10 @@ class SoAForm : public Form {
  public:
   SoAForm(int n_inputs, int n_outputs) : n_in_(n_inputs), n_out_(n_outputs) {}
   ~SoAForm() override {
-    if (table_) (void)hipFree(table_);
+    for (auto& t : tables_) (void)hipFree(t.second);
     if (start_) (void)hipEventDestroy(start_);
     if (stop_) (void)hipEventDestroy(stop_);
+    if (done_) (void)hipEventDestroy(done_);
   }
   int n_inputs() const override { return n_in_; }
   int n_outputs() const override { return n_out_; }
@@ -41,6 +43,7 @@ class SoAForm : public Form {
       for (int i = 0; i < n_out_; i++) device::check(hipMalloc(&dout[i], n * sizeof(double)), "hipMalloc");
       run(din.data(), n, dout.data(), stream, kernel_ms);
       device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      forget_table(din.data(), dout.data());  // the staging arrays are freed on return
       for (int i = 0; i < n_out_; i++)
         device::check(hipMemcpy(outputs[i], dout[i], n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
       return;
@@ -52,25 +55,74 @@ class SoAForm : public Form {
   // in/out: device tables of device pointers.
   virtual void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) = 0;
 
- private:
-  int n_in_, n_out_;
-  void* table_ = nullptr;
-  std::vector<const void*> last_;
-  hipEvent_t start_ = nullptr, stop_ = nullptr;
+  // Orders this call after the module's previous call when the two come on different streams, so that state the
+  // module keeps between its kernels (the surrogate's fallback list) is never shared by two calls in flight. Not
+  // while the stream is being captured into a graph: the caller orders captured work.
+  void order_after_previous_call(hipStream_t stream) {
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    device::check(hipStreamIsCapturing(stream, &capture), "hipStreamIsCapturing");
+    if (capture != hipStreamCaptureStatusNone) return;
+    if (done_ && stream != last_stream_) device::check(hipStreamWaitEvent(stream, done_, 0), "hipStreamWaitEvent");
+  }
+  void mark_call_done(hipStream_t stream) {
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    device::check(hipStreamIsCapturing(stream, &capture), "hipStreamIsCapturing");
+    if (capture != hipStreamCaptureStatusNone) return;
+    if (!done_) device::check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreateWithFlags");
+    device::check(hipEventRecord(done_, stream), "hipEventRecord");
+    last_stream_ = stream;
+  }
 
-  void run(const double* const* inputs, size_t n, double* const* outputs, hipStream_t stream, float* kernel_ms) {
+ private:
+  // One device table of the arrays' pointers per distinct set of arrays, written once when the set is first seen and
+  // never rewritten while the module lives: a launch still in flight on any stream, or a graph captured from this
+  // module, keeps reading the table it was given. Past kMaxTables sets the device is synchronized and the tables
+  // are freed (graphs captured before that point must then be captured again).
+  static constexpr size_t kMaxTables = 1024;
+  int n_in_, n_out_;
+  std::map<std::vector<const void*>, void*> tables_;
+  hipEvent_t start_ = nullptr, stop_ = nullptr, done_ = nullptr;
+  hipStream_t last_stream_ = nullptr;
+
+  void* table_for(const std::vector<const void*>& ptrs) {
+    auto it = tables_.find(ptrs);
+    if (it != tables_.end()) return it->second;
+    if (tables_.size() >= kMaxTables) {
+      device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      for (auto& t : tables_) device::check(hipFree(t.second), "hipFree table");
+      tables_.clear();
+    }
+    void* table = nullptr;
+    device::check(hipMalloc(&table, ptrs.size() * sizeof(void*)), "hipMalloc table");
+    const hipError_t e = hipMemcpy(table, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(table);
+      device::check(e, "hipMemcpy table");
+    }
+    tables_.emplace(ptrs, table);
+    return table;
+  }
+
+  std::vector<const void*> pointer_set(const double* const* inputs, double* const* outputs) const {
     std::vector<const void*> ptrs;
     for (int i = 0; i < n_in_; i++) ptrs.push_back(inputs[i]);
     for (int i = 0; i < n_out_; i++) ptrs.push_back(outputs[i]);
-    if (!table_) device::check(hipMalloc(&table_, ptrs.size() * sizeof(void*)), "hipMalloc table");
-    if (ptrs != last_) {
-      // The table may still be read by an earlier launch on this stream.
-      device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      device::check(hipMemcpy(table_, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice), "hipMemcpy table");
-      last_ = ptrs;
-    }
-    auto in = (const double* const*)table_;
-    auto out = (double* const*)((const void**)table_ + n_in_);
+    return ptrs;
+  }
+
+  // Frees the table of a pointer set whose launches have completed.
+  void forget_table(const double* const* inputs, double* const* outputs) {
+    auto it = tables_.find(pointer_set(inputs, outputs));
+    if (it == tables_.end()) return;
+    device::check(hipFree(it->second), "hipFree table");
+    tables_.erase(it);
+  }
+
+  void run(const double* const* inputs, size_t n, double* const* outputs, hipStream_t stream, float* kernel_ms) {
+    void* table = table_for(pointer_set(inputs, outputs));
+    auto in = (const double* const*)table;
+    auto out = (double* const*)((const void**)table + n_in_);
+    order_after_previous_call(stream);
     if (kernel_ms) {
       if (!start_) {
         device::check(hipEventCreate(&start_), "hipEventCreate");
@@ -79,10 +131,12 @@ class SoAForm : public Form {
       device::check(hipEventRecord(start_, stream), "hipEventRecord");
       launch(in, out, n, stream);
       device::check(hipEventRecord(stop_, stream), "hipEventRecord");
+      mark_call_done(stream);
       device::check(hipEventSynchronize(stop_), "hipEventSynchronize");
       device::check(hipEventElapsedTime(kernel_ms, start_, stop_), "hipEventElapsedTime");
     } else {
       launch(in, out, n, stream);
+      mark_call_done(stream);
     }
   }
 };

@@ -146,7 +146,7 @@ MixedTables mixed_tables(const Mechanism& m) {
       t.troe[4 * r] = (float)x.troe.A;
       t.troe[4 * r + 1] = (float)(1. / x.troe.T3);
       t.troe[4 * r + 2] = x.troe.T1 < 1e30 ? (float)(1. / x.troe.T1) : 0.f;
-      t.troe[4 * r + 3] = std::isfinite(x.troe.T2) ? (float)x.troe.T2 : 0.f;
+      t.troe[4 * r + 3] = std::isfinite(x.troe.T2) ? (float)x.troe.T2 : NAN;  // NaN: no T2 term (a finite T2, 0 included, adds exp(-T2/T))
     }
     t.slots[4 * r] = type | (kinf_const << 4);
     std::vector<int> offsets;
@@ -326,8 +326,16 @@ void check_program(const ProgramView& p) {
   if (p.chemistry->options.rhs) throw Unsupported("tiled forms do not evaluate the reactor right-hand side");
 }
 
+constexpr size_t kMaxStraightRatesSpecies = 64;
+
 // The rates section of a chemistry program as straight-line FP64 code (contraction and reciprocal division).
 std::string rates_straight_text(const ProgramView& p, bool fast) {
+  // Above 64 species the straight-line rates kernel spills kilobytes per lane to scratch (6.9 KB at K=160) and its
+  // code object runs to megabytes; the one run of the K=160 FP64 form on a GPU ended in a memory aperture violation
+  // during assembly whose cause was not established, so these mechanisms are refused here.
+  if (p.mechanism->species.len() > kMaxStraightRatesSpecies)
+    throw Unsupported("the straight-line FP64 rates kernel supports at most " + std::to_string(kMaxStraightRatesSpecies) +
+                      " species (this mechanism has " + std::to_string(p.mechanism->species.len()) + ")");
   ChemistryOptions o;
   o.transport = false;
   o.lean_rates = true;
@@ -412,7 +420,8 @@ class RatesStraight {
 // Mechanisms up to 64 species take the transport section as straight-line code (one lane per state): at K=53 it
 // issues fewer instructions per state than the tiled kernel, whose MFMA contractions do not pay for the
 // reciprocals and reductions around them. Larger mechanisms (a K² program) take transport_tiled.
-bool straight_transport(const ProgramView& p) { return p.mechanism->species.len() <= 64; }
+// form | kFormTiledTransport sends every mechanism through transport_tiled (the MFMA contractions) instead.
+bool straight_transport(const ProgramView& p, int form) { return p.mechanism->species.len() <= 64 && !(form & kFormTiledTransport); }
 
 std::string transport_straight_text(const ProgramView& p, bool f64) {
   ChemistryOptions o;
@@ -450,9 +459,10 @@ class TransportStraight {
 class Tiled : public SoAForm {
  public:
   // form 1: FP64 rates and transport; 2: MFMA mixed rates, FP32 transport; 3: FP32-core rates, FP32 transport.
-  Tiled(const ProgramView& p, int form) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
+  Tiled(const ProgramView& p, int form_flags) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
     check_program(p);
     const auto& o = p.chemistry->options;
+    const int form = form_flags & kFormBase;
     const bool f64 = form == 1;
     if (o.rates) {
       n_rates_ = (int)p.mechanism->active;
@@ -462,7 +472,7 @@ class Tiled : public SoAForm {
         fp64_ = std::make_unique<RatesStraight>(p, form == 3);
     }
     if (o.transport) {
-      if (straight_transport(p))
+      if (straight_transport(p, form_flags))
         straight_ = std::make_unique<TransportStraight>(p, f64);
       else
         transport_ = std::make_unique<TransportTiled>(p, f64);
@@ -634,6 +644,8 @@ class Surrogate : public SoAForm {
  protected:
   void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
     if (n > capacity_) {
+      // an earlier call on another stream may still fill the list
+      device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       if (d_fallback_) device::check(hipFree(d_fallback_), "hipFree");
       d_fallback_ = nullptr;
       device::check(hipMalloc(&d_fallback_, n * sizeof(unsigned)), "hipMalloc");
@@ -693,19 +705,21 @@ class Surrogate : public SoAForm {
 
 bool tiled_applies(const ProgramView& p) { return p.kind == ProgramKind::Chemistry && !p.chemistry->options.rhs; }
 
-std::string tiled_source(const ProgramView& p, int form, unsigned) {
+std::string tiled_source(const ProgramView& p, int form_flags, unsigned) {
   check_program(p);
+  const int form = form_flags & kFormBase;
   const bool mixed = form != 1;
   std::string s;
   if (p.chemistry->options.rates) s += form == 2 ? rates_mixed_text(mixed_tables(*p.mechanism)) : rates_straight_text(p, form == 3);
   if (p.chemistry->options.transport)
-    s += "\n// ---- transport ----\n" + (straight_transport(p) ? transport_straight_text(p, !mixed)
+    s += "\n// ---- transport ----\n" + (straight_transport(p, form_flags) ? transport_straight_text(p, !mixed)
                                                          : transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed));
   return s;
 }
 
-void compile_tiled(const ProgramView& p, int form, unsigned) {
+void compile_tiled(const ProgramView& p, int form_flags, unsigned) {
   check_program(p);
+  const int form = form_flags & kFormBase;
   const bool mixed = form != 1;
   if (p.chemistry->options.rates) {
     if (form == 2)
@@ -714,7 +728,7 @@ void compile_tiled(const ProgramView& p, int form, unsigned) {
       device::compile(rates_straight_text(p, form == 3), form == 3 ? "rates_fast" : "rates_fp64", hip_options(true));
   }
   if (p.chemistry->options.transport) {
-    if (straight_transport(p))
+    if (straight_transport(p, form_flags))
       device::compile(transport_straight_text(p, !mixed), mixed ? "transport_f32" : "transport_fp64", hip_options(true));
     else
       device::compile(transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed), "transport_tiled",

@@ -22,6 +22,17 @@ LIB_PATH = os.path.join(HERE, "libcombustion_amd.so")
 RATES, TRANSPORT, RHS, LEAN = 1, 2, 4, 8
 FORM_STRAIGHT, FORM_FP64, FORM_MIXED, FORM_FAST = 0, 1, 2, 3
 FORMS = {"straight": FORM_STRAIGHT, "fp64": FORM_FP64, "mixed": FORM_MIXED, "fast": FORM_FAST}
+FORM_TILED_TRANSPORT = 0x10  # cb_form_flags: "fp64+tiled" etc. send the transport section through the MFMA kernel
+
+
+def form_code(form):
+    """cb_hip_assemble's form argument from a name: "straight", "fp64", "mixed", "fast", optionally with "+tiled"."""
+    if isinstance(form, int):
+        return form
+    base, _, modifier = form.partition("+")
+    if base not in FORMS or modifier not in ("", "tiled"):
+        raise ValueError(f"unknown form {form!r}")
+    return FORMS[base] | (FORM_TILED_TRANSPORT if modifier else 0)
 
 STATUS = {0: "ok", -1: "argument", -2: "parse", -3: "unsupported", -4: "build", -5: "compile", -6: "device", -7: "size"}
 
@@ -49,6 +60,8 @@ SIGNATURES = {
     "cb_mechanism_free": (None, [_P]),
     "cb_mechanism_sizes": (C.c_int, [_P, _I32P, _I32P, _I32P]),
     "cb_mechanism_json": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cb_ignition_state_load": (C.c_int, [_P, C.c_char_p, _DP, _DP, _DP]),
+    "cb_ignition_state_parse": (C.c_int, [_P, C.c_char_p, _DP, _DP, _DP]),
     "cb_transport_fits": (C.c_int, [_P, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "cb_transport_species": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_double, _DP]),
     "cb_transport_scales": (C.c_int, [_P, C.c_double, _DP, _DP]),
@@ -65,6 +78,7 @@ SIGNATURES = {
     "cb_program_hip_source": (C.c_int, [_P, C.c_int32, C.c_uint32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "cb_export_device_function": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "cb_hip_assemble": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
+    "cb_hip_assemble_export": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
     "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
     "cb_module_free": (None, [_P]),
@@ -151,6 +165,15 @@ class Mechanism:
     def molar_mass(self):
         return np.array([s["molar_mass"] for s in self.tables["species"]])
 
+    def ignition_state(self, path=None, text=None):
+        """cantera.rs:13-29: (T [K], P [Pa], amounts [mol] in V = 1 m³) of the first state of a .ignition file."""
+        T, P, n = C.c_double(), C.c_double(), np.zeros(self.K)
+        if text is not None:
+            _check(lib().cb_ignition_state_parse(self._h, text.encode(), C.byref(T), C.byref(P), n.ctypes.data_as(_DP)))
+        else:
+            _check(lib().cb_ignition_state_load(self._h, os.fsencode(path), C.byref(T), C.byref(P), n.ctypes.data_as(_DP)))
+        return T.value, P.value, n
+
     def transport_fits(self, D=4, T0=1000.0):
         K = self.K
         c, v, d = np.zeros((K, D)), np.zeros((K, D)), np.zeros((K, K, D))
@@ -204,6 +227,16 @@ class Mechanism:
                 f.write(self.device_function(which, prefix, D))
             paths.append(path)
         return paths
+
+    def export_module(self, D=5, compile_only=False):
+        """cb_hip_assemble_export: the four exported files compiled by hiprtc behind one SoA kernel (inputs [T, P, Y],
+        outputs [ω, λ, μ, D_k]); a Module, or None after filling the code-object cache when compile_only is set."""
+        if compile_only:
+            _check(lib().cb_hip_assemble_export(self._h, D, None))
+            return None
+        h = _P()
+        _check(lib().cb_hip_assemble_export(self._h, D, C.byref(h)))
+        return Module(h, _Signature(self.K + 2, self.active + 2 + self.K))
 
     def chemistry(self, flags=RATES | TRANSPORT, D=4, T0=1000.0, viscosity=0.0, conductivity=0.0):
         """Per-state SoA program: inputs [T, P, Y_0..Y_{K-1}] -> [ω_0..ω_{Ka-1}, λ, μ, D_0..D_{K-1}, (dT/dt)]."""
@@ -264,17 +297,24 @@ class Program:
         return cls(h, None)
 
     def hip_source(self, form="straight", block_size=0):
-        return _text(lib().cb_program_hip_source, self._h, FORMS[form], block_size)
+        return _text(lib().cb_program_hip_source, self._h, form_code(form), block_size)
 
     def compile(self, form="straight", block_size=0):
         """Fills the code-object cache for a form (hiprtc for gfx950, no GPU needed)."""
-        _check(lib().cb_hip_compile(self._h, FORMS[form], block_size))
+        _check(lib().cb_hip_compile(self._h, form_code(form), block_size))
 
     def assemble(self, block_size=0, form="straight"):
         """device::assemble (device.rs:25): emit, compile with hiprtc, load the module."""
         h = _P()
-        _check(lib().cb_hip_assemble(self._h, FORMS[form], block_size, C.byref(h)))
+        _check(lib().cb_hip_assemble(self._h, form_code(form), block_size, C.byref(h)))
         return Module(h, self)
+
+
+class _Signature:
+    """Input and output counts of a module that was not built from a Program."""
+
+    def __init__(self, n_inputs, n_outputs):
+        self.n_inputs, self.n_outputs = n_inputs, n_outputs
 
 
 def _pointer(x):

@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import itertools
 import math
+import re
 import os
 
 import numpy as np
@@ -376,3 +377,81 @@ def states(K, n, seed, T_range=(600.0, 2800.0), P_range_atm=(1.0, 40.0), alpha=0
 def config_states(name, n, seed):
     (T_range, P_range, alpha) = STATE_RANGES[name]
     return states(CONFIGS[name][0], n, seed, T_range, P_range, alpha)
+
+
+def sri_variant(name, seed=5, params=None):
+    """YAML text of a named synthetic mechanism with every Troe block replaced by an SRI block: drawn
+    (a ~ U[0.5, 2], b ~ U[100, 1000] K, c ~ U[500, 3000] K, d ~ U[0.8, 1.2], e ~ U[-0.5, 0.5]; the first one with
+    c = 0), or the given (a, b, c, d, e) for all."""
+    text = open(mechanism_path(name)).read()
+    rng = np.random.default_rng(seed)
+    count = [0]
+
+    def block(_):
+        if params is not None:
+            a, b, c, d, e = params
+        else:
+            a, b, c, d, e = rng.uniform(0.5, 2), rng.uniform(100, 1000), rng.uniform(500, 3000), rng.uniform(0.8, 1.2), rng.uniform(-0.5, 0.5)
+            if count[0] == 0:
+                c = 0.0
+        count[0] += 1
+        return f"SRI: {{A: {a!r}, B: {b!r}, C: {c!r}, D: {d!r}, E: {e!r}}}"
+    return re.sub(r"Troe: \{[^}]*\}", block, text)
+
+
+def plog_variant(name, every=3, seed=6):
+    """YAML text of a named synthetic mechanism with every `every`-th elementary reaction turned into a PLOG table:
+    entries at 2, 8 and 30 atm (its rate constant scaled by drawn factors and exponents), plus a second entry at
+    30 atm that the table sums."""
+    text = open(mechanism_path(name)).read()
+    rng = np.random.default_rng(seed)
+    count = [0]
+
+    def table(m):
+        count[0] += 1
+        if count[0] % every:
+            return m.group(0)
+        A, b, Ea = float(m.group(2)), float(m.group(3)), float(m.group(4))
+        rows = []
+        for P in ("2.0 atm", "8.0 atm", "30.0 atm", "30.0 atm"):
+            f, db = rng.uniform(0.2, 5.0), rng.uniform(-0.3, 0.3)
+            rows.append(f"  - {{P: {P}, A: {A * f!r}, b: {b + db!r}, Ea: {Ea * rng.uniform(0.8, 1.2)!r}}}")
+        return f"{m.group(1)}\n  type: pressure-dependent-Arrhenius\n  rate-constants:\n" + "\n".join(rows)
+    return re.sub(r"(- equation: [^\n]*<=>[^\n(]*)\n  rate-constant: \{A: ([^,]*), b: ([^,]*), Ea: ([^}]*)\}", table, text)
+
+def models_variant(name, seed=7):
+    """YAML text of a named synthetic mechanism that exercises the rate models the synthetic draws leave out:
+    of the falloff reactions, every third loses its Troe block (Lindemann falloff, reaction.rs:153-158) and every
+    third from the second gets a constant high-pressure limit (b = Ea = 0: the Troe branch of reaction.rs:172-174);
+    every fifth elementary reaction becomes irreversible (`=>`, reaction.rs:272). The seed is kept for the
+    signature's sake."""
+    del seed
+    text = open(mechanism_path(name)).read()
+    count = {"falloff": 0, "elementary": 0}
+
+    def falloff(m):
+        i = count["falloff"]
+        count["falloff"] += 1
+        block = m.group(0)
+        if i % 3 == 0:
+            return re.sub(r"\n  Troe: \{[^}]*\}", "", block)
+        if i % 3 == 1:
+            block = re.sub(r"high-P-rate-constant: \{A: ([^,]*), b: [^,]*, Ea: [^}]*\}",
+                           lambda h: f"high-P-rate-constant: {{A: {h.group(1)}, b: 0.0, Ea: 0.0}}", block)
+        return block
+
+    text = re.sub(r"- equation: [^\n]*\(\+M\)[^\n]*\n  type: falloff\n(?:  [^\n]*\n)*?  Troe: \{[^}]*\}", falloff, text)
+
+    def elementary(m):
+        count["elementary"] += 1
+        return m.group(0).replace(" <=> ", " => ") if count["elementary"] % 5 == 0 else m.group(0)
+
+    return re.sub(r"- equation: [^\n(]*<=>[^\n(]*\n  rate-constant:", elementary, text)
+
+
+VARIANTS = {"sri": sri_variant, "plog": plog_variant, "models": models_variant}
+
+
+def variant_text(variant, name):
+    """The YAML text of a named variant ("sri", "plog", "models") of a named synthetic mechanism."""
+    return VARIANTS[variant](name)

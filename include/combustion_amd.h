@@ -44,6 +44,10 @@ enum cb_form {
   CB_FORM_MIXED = 2,    /* tiled mixed precision: BF16-split MFMA contractions, FP32 exp/log, FP64 ΔG/RT */
   CB_FORM_FAST = 3      /* FP64 straight-line rates with FP32 cores for exp and ln, FP32 transport */
 };
+/* Modifier ORed into the form of a chemistry program (forms 1-3): the transport section takes the tiled MFMA
+ * kernel (Wilke sums and binary-diffusion polynomials as v_mfma_{f64,f32}_16x16x4 contractions) at any species
+ * count, instead of the straight-line transport kernel that forms 1 and 3 use up to 64 species. */
+enum cb_form_flags { CB_FORM_TILED_TRANSPORT = 0x10 };
 
 /* Sections of the per-state program built by cb_build_chemistry. CB_LEAN selects the shapes that the
  * one-lane-per-state FP64 kernels compile (the same values to a few ulps per operation): rate-of-progress products
@@ -63,6 +67,14 @@ void cb_mechanism_free(cb_mechanism* m);
 int cb_mechanism_sizes(const cb_mechanism* m, int32_t* species, int32_t* active, int32_t* reactions);
 /* JSON dump of the parsed tables in SI units (lib.rs:26-125 fields), for inspection and tests. */
 int cb_mechanism_json(const cb_mechanism* m, char* buffer, size_t capacity, size_t* needed);
+
+/* cantera.rs:13-29: the initial state of a `<mechanism>.ignition` file, a YAML sequence whose first entry holds
+ * `pressure` [Pa], `temperature` [K] and `X`, a map from every species name to its mole proportion (>= 0, any
+ * normalisation), and no `volume`. Outputs the reference's State: T, P and the amounts in V = 1 m³,
+ * n_k = (P/R) V / T · x_k / Σx (amounts[K], the mechanism's species order). A missing species, a negative
+ * proportion or a given volume (all assertions in the reference) return CB_ERR_PARSE. */
+int cb_ignition_state_load(const cb_mechanism* m, const char* path, double* temperature, double* pressure, double* amounts);
+int cb_ignition_state_parse(const cb_mechanism* m, const char* text, double* temperature, double* pressure, double* amounts);
 
 /* transport.rs:125-136 Polynomials::<D>::new: fits in ln(T/T0), conductivity[K][D], viscosity[K][D],
  * diffusivity[K][K][D] (entries j<k fitted, j>k mirrored). */
@@ -110,6 +122,13 @@ int cb_program_hip_source(const cb_program* p, int32_t form, uint32_t block_size
 enum cb_export { CB_EXPORT_RATES = 0, CB_EXPORT_CONDUCTIVITY = 1, CB_EXPORT_VISCOSITY = 2, CB_EXPORT_DIFFUSIVITY = 3 };
 int cb_export_device_function(const cb_mechanism* m, int32_t which, int32_t D, const char* prefix, char* buffer, size_t capacity,
                                size_t* needed);
+/* The four exported files (prefix nekrk_, fits of length D) compiled together by hiprtc for gfx950 (dfloat = double,
+ * __NEKRK_DEVICE__ = a device function) behind one kernel over SoA states, called with cb_hip_call: inputs
+ * [T, P, Y_0..Y_{K-1}], outputs [ω_0..ω_{Ka-1}, λ, μ, D_0..D_{K-1}] in SI units, the functions' arguments formed as the
+ * reference's check forms them (cuda.rs:350-369: mole proportions = concentrations, powers of ln(T/T0), √(T/T0)).
+ * It runs the exported text itself on the device, as a CFD code including the files would. With out == NULL the
+ * code object is only compiled into the cache (no GPU needed). */
+int cb_hip_assemble_export(const cb_mechanism* m, int32_t D, cb_module** out);
 
 /* device.rs:75 assemble: emit + hiprtc(gfx950) + module load. block_size 0 picks the form's default. */
 int cb_hip_assemble(const cb_program* p, int32_t form, uint32_t block_size, cb_module** out);

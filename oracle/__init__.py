@@ -45,8 +45,9 @@ def lib():
         _lib.oracle_batch.restype = C.c_int
         _lib.oracle_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                       C.c_void_p, C.c_void_p, C.c_int]
-        _lib.oracle_reaction_rates.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
-        _lib.oracle_rhs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.oracle_reaction_rates.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.oracle_rhs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
         _lib.interpret.restype = C.c_long
         _lib.interpret.argtypes = [C.c_void_p] * 6 + [C.c_long, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                                       C.c_long, C.POINTER(C.c_long)]
@@ -106,18 +107,26 @@ class Oracle:
             raise RuntimeError("oracle_batch failed")
         return out, scale
 
-    def reaction_rates(self, T, P, Y):
-        q = np.zeros(self.m["R"])
+    def reaction_rates(self, T, P, Y, scale=False):
+        """Rates of progress q_r of one state (and their scales |k_f| (R_f + R_r) when scale is set)."""
+        q, qs = np.zeros(self.m["R"]), np.zeros(self.m["R"])
         Y = np.ascontiguousarray(Y, np.float64)
-        lib().oracle_reaction_rates(C.byref(self.s), float(T), float(P), Y.ctypes.data, q.ctypes.data)
-        return q
+        lib().oracle_reaction_rates(C.byref(self.s), float(T), float(P), Y.ctypes.data, q.ctypes.data, qs.ctypes.data)
+        return (q, qs) if scale else q
 
     def rhs(self, T, P, Y):
-        omega = np.zeros(self.m["Ka"])
-        dT = C.c_double()
+        omega, dT, _ = self.rhs_full(T, P, Y)
+        return omega, dT
+
+    def rhs_full(self, T, P, Y):
+        """(ω, dT/dt, extras) of one state; extras holds the term scales of ω (omega_scale), the scale of dT/dt
+        (dTdt_scale), the relative volume rate (dV/dt)/V at constant pressure (dlnV) and its scale (dlnV_scale)."""
+        omega, scale = np.zeros(self.m["Ka"]), np.zeros(self.m["Ka"])
+        v = [C.c_double() for _ in range(4)]
         Y = np.ascontiguousarray(Y, np.float64)
-        lib().oracle_rhs(C.byref(self.s), float(T), float(P), Y.ctypes.data, omega.ctypes.data, C.byref(dT))
-        return omega, dT.value
+        lib().oracle_rhs(C.byref(self.s), float(T), float(P), Y.ctypes.data, omega.ctypes.data, scale.ctypes.data, *[C.byref(x) for x in v])
+        dT, dT_scale, dlnV, dlnV_scale = (x.value for x in v)
+        return omega, dT, {"omega_scale": scale, "dTdt_scale": dT_scale, "dlnV": dlnV, "dlnV_scale": dlnV_scale}
 
 
 def interpret(linear, inputs):

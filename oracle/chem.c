@@ -112,8 +112,10 @@ static double concentrations(const oracle_mechanism* m, double T, double P, cons
   return C_tot;
 }
 
-/* ω_k (k < Ka), their term scales Σ_r |ν_kr| k_f (R_f + R_r), optionally q_r, and dT/dt. */
-void oracle_rates(const oracle_mechanism* m, double T, double P, const double* Y, double* omega, double* scale, double* q_out, double* dTdt) {
+/* ω_k (k < Ka), their term scales Σ_r |ν_kr| k_f (R_f + R_r), optionally q_r and its scale |k_f| (R_f + R_r), and dT/dt
+   (reaction.rs:375-380) with its scale T Σ_k s_k |h_k/RT| / Σ_k cp_k/R C_k (s_k the term scale of ω_k). */
+void oracle_rates(const oracle_mechanism* m, double T, double P, const double* Y, double* omega, double* scale, double* q_out,
+                  double* q_scale, double* dTdt, double* dTdt_scale) {
   const int K = m->K, Ka = m->Ka, R = m->R;
   double C[1024], Ep[1024], En[1024];
   const double C_tot = concentrations(m, T, P, Y, C);
@@ -187,6 +189,7 @@ void oracle_rates(const oracle_mechanism* m, double T, double P, const double* Y
     }
     double q = kf * (Rf - Rr);
     if (q_out) q_out[r] = q;
+    if (q_scale) q_scale[r] = fabs(kf) * (Rf + Rr);
     for (int k = 0; k < Ka; k++) {
       int nu = nu_r[k] - nu_f[k];
       if (!nu) continue;
@@ -195,10 +198,15 @@ void oracle_rates(const oracle_mechanism* m, double T, double P, const double* Y
     }
   }
   if (dTdt) {
-    double num = 0, den = 0;
-    for (int k = 0; k < Ka; k++) num += omega[k] * enthalpy_RT(m, k, T);
+    double num = 0, num_scale = 0, den = 0;
+    for (int k = 0; k < Ka; k++) {
+      const double h = enthalpy_RT(m, k, T);
+      num += omega[k] * h;
+      num_scale += scale[k] * fabs(h);
+    }
     for (int k = 0; k < K; k++) den += cp_R(m, k, T) * C[k];
     *dTdt = -T * num / den;
+    if (dTdt_scale) *dTdt_scale = T * num_scale / fabs(den);
   }
 }
 
@@ -276,7 +284,7 @@ static void* run(void* arg) {
     for (int k = 0; k < K; k++) Y[k] = j->Y[(size_t)k * j->n + i];
     int o = 0;
     if (j->rates) {
-      oracle_rates(m, j->T[i], j->P[i], Y, omega, sc, NULL, NULL);
+      oracle_rates(m, j->T[i], j->P[i], Y, omega, sc, NULL, NULL, NULL, NULL);
       for (int k = 0; k < Ka; k++, o++) {
         j->out[o][i] = omega[k];
         if (j->scale) j->scale[o][i] = sc[k];
@@ -320,16 +328,32 @@ int oracle_batch(const oracle_mechanism* m, int rates, int transport, const doub
 }
 
 /* Per-reaction rates of progress (reaction.rs:329 reactions_rates) for one state. */
-int oracle_reaction_rates(const oracle_mechanism* m, double T, double P, const double* Y, double* q) {
+int oracle_reaction_rates(const oracle_mechanism* m, double T, double P, const double* Y, double* q, double* q_scale) {
   double omega[1024], scale[1024];
   if (m->K > 1024) return -1;
-  oracle_rates(m, T, P, Y, omega, scale, q, NULL);
+  oracle_rates(m, T, P, Y, omega, scale, q, q_scale, NULL, NULL);
   return 0;
 }
 
-int oracle_rhs(const oracle_mechanism* m, double T, double P, const double* Y, double* omega, double* dTdt) {
-  double scale[1024];
+/* ω (and term scales), dT/dt (and scale), and the relative volume rate (dV/dt)/V of the constant-pressure reactor
+   (reaction.rs:381-385): dT/dt / T + (R/P) T Σ_k (1 - W_k/W_bulk) ω_k, with scale |dT/dt|_scale / T + (R/P) T Σ_k
+   |1 - W_k/W_bulk| s_k. */
+int oracle_rhs(const oracle_mechanism* m, double T, double P, const double* Y, double* omega, double* scale, double* dTdt,
+               double* dTdt_scale, double* dlnV, double* dlnV_scale) {
+  double own_scale[1024];
   if (m->K > 1024) return -1;
-  oracle_rates(m, T, P, Y, omega, scale, NULL, dTdt);
+  if (!scale) scale = own_scale;
+  double dT_scale = 0;
+  oracle_rates(m, T, P, Y, omega, scale, NULL, NULL, dTdt, &dT_scale);
+  if (dTdt_scale) *dTdt_scale = dT_scale;
+  const double Wb = m->W[m->K - 1];
+  double s = 0, s_scale = 0;
+  for (int k = 0; k < m->Ka; k++) {
+    s += (1. - m->W[k] / Wb) * omega[k];
+    s_scale += fabs(1. - m->W[k] / Wb) * scale[k];
+  }
+  const double T_pR = T * RGAS / P;
+  if (dlnV) *dlnV = *dTdt / T + T_pR * s;
+  if (dlnV_scale) *dlnV_scale = dT_scale / T + T_pR * s_scale;
   return 0;
 }

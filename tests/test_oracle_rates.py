@@ -250,3 +250,19 @@ def test_rhs_energy_balance_sign():
     q = 1e10 * C[i["B"]]
     expect = -(h("D") - h("B")) * q / sum(cp(n) * C[i[n]] for n in THERMO)
     assert dT == pytest.approx(expect, rel=1e-12)
+
+
+def test_rhs_volume_rate_is_the_ideal_gas_law():
+    """(dV/dt)/V of reaction.rs:381-385 weights each ω_k by 1 - W_k/W_bulk; with element-balanced reactions
+    (Σ_k W_k ω_k = 0) that is the ideal-gas law at constant pressure, d ln V = d ln T + d ln n with
+    d n/dt = V Σ_k ω_k and n = P V / (R T). A weight on the wrong side (W_bulk/W_k) or a dropped term fails it."""
+    from combustion_amd import synth
+    from helpers import setup
+    M, mech, o = setup("gri-53")
+    T, P, Y = synth.config_states("gri-53", 16, 77)
+    for s in range(16):
+        omega, dT, x = o.rhs_full(T[s], P[s], Y[:, s])
+        expect = dT / T[s] + T[s] * R / P[s] * omega.sum()
+        bound = x["dlnV_scale"] + np.abs(omega).sum() * T[s] * R / P[s]
+        assert abs(x["dlnV"] - expect) <= 1e-12 * bound
+        assert x["dTdt_scale"] >= abs(dT) * (1 - 1e-12)
