@@ -33,25 +33,52 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(name, budget_s=12.0):
-    """The FP64 oracle (a C restatement of the reference's per-state evaluation) on the host's cores."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from helpers import setup
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prog, name, budget_s=8.0):
+    """The reference's CPU path as planned for the baseline: the per-state program compiled by the host compiler
+    (cb_cpu_assemble, the counterpart of ir/lib.rs:225-263) and called once per state as device.rs:35-45 does, timed
+    on one core and on the box's host cores (at most 16, the GPU's share of them); the FP64 oracle for information."""
     from combustion_amd import synth
-    _, _, o = setup(name)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    T, P, Y = synth.config_states(name, 256, 99)
+    mod = prog.assemble_cpu()
+    T, P, Y = synth.config_states(name, 512, 99)
     t = time.perf_counter()
-    o.batch(T, P, Y, threads=1)
-    single = 256 / (time.perf_counter() - t)
-    n = int(max(1024, min(1 << 20, single * threads * budget_s)))
+    mod([T, P, *Y], threads=1)
+    probe = 512 / (time.perf_counter() - t)
+    n1 = int(max(1024, min(1 << 20, probe * budget_s / 2)))
+    T, P, Y = synth.config_states(name, n1, 99)
+    t = time.perf_counter()
+    mod([T, P, *Y], threads=1)
+    single = n1 / (time.perf_counter() - t)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = int(max(1024, min(1 << 20, single * threads * budget_s / 2)))
     T, P, Y = synth.config_states(name, n, 99)
     t = time.perf_counter()
-    o.batch(T, P, Y, threads=threads)
+    mod([T, P, *Y], threads=threads)
     rate = n / (time.perf_counter() - t)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import setup
+    _, _, o = setup(name)
+    T, P, Y = synth.config_states(name, 256, 98)
+    t = time.perf_counter()
+    o.batch(T, P, Y, threads=1)
+    oracle_rate = 256 / (time.perf_counter() - t)
     return {"value": rate, "unit": "states/s", "cores": threads, "kind": "port",
-            "sample": f"{n} states of the same workload (rates + transport, K=53/R=325), FP64 oracle (oracle/chem.c) on "
-                      f"{threads} host threads; single-thread {single:.0f} states/s"}
+            "single_core": single, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"{n} states (1 core: {n1}) of the same workload (rates + transport, K=53/R=325): the per-state FP64 "
+                      f"program emitted as C, compiled by host clang -O2 (cb_cpu_assemble, the counterpart of the "
+                      f"reference's Cranelift JIT ir/lib.rs:225-263) and called once per state as device.rs:35-45 does, on "
+                      f"{threads} host threads over contiguous state ranges; the FP64 oracle (oracle/chem.c) runs "
+                      f"{oracle_rate:.0f} states/s on one core"}
 
 
 def traffic_from_profiles(kernel):
@@ -229,7 +256,7 @@ def main():
         result["surrogate_workload"] = surrogate_workload(args, M, prog, dev)
     result["forms"] = forms
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(MECHANISM)
+        result["cpu_baseline"] = cpu_baseline(prog, MECHANISM)
     print(json.dumps(result))
 
 

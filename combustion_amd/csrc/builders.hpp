@@ -81,6 +81,9 @@ struct ChemistryOptions {
   bool f32_range = false;
 };
 ast::Function chemistry(const Mechanism& m, const Polynomials* p, const TransportScales& scales, const ChemistryOptions& o);
+// Constant-pressure reactor right-hand side: inputs [T, P, Y_0..Y_{K-1}], outputs [dT/dt, dY_0/dt..dY_{K-1}/dt]
+// (dY_k/dt = W_k ω_k / ρ, zero for species that do not react); lean as ChemistryOptions::lean_rates.
+ast::Function reactor(const Mechanism& m, bool lean = false);
 // The registers of a linearized rates program that hold the clamped exp(-G_k/RT) (negative) or exp(G_k/RT) values
 // (reaction.rs:236-268), in order of definition.
 std::vector<int32_t> gibbs_exponentials(const ast::Linear& l, bool negative);

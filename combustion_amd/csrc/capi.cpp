@@ -11,7 +11,10 @@
 #include "device.hpp"
 #include "emit.hpp"
 #include "export.hpp"
+#include "host_jit.hpp"
 #include "ignition.hpp"
+#include "integrator.hpp"
+#include "jacobian.hpp"
 #include "forms.hpp"
 #include "mechanism.hpp"
 #include "mlp.hpp"
@@ -33,6 +36,14 @@ struct cb_program {
 
 struct cb_module {
   std::unique_ptr<Form> form;
+};
+
+struct cb_cpu_module {
+  std::unique_ptr<HostProgram> program;
+};
+
+struct cb_integrator {
+  std::unique_ptr<Integrator> integrator;
 };
 
 struct cb_mlp {
@@ -317,6 +328,28 @@ int cb_build_chemistry(const cb_mechanism* m, int32_t flags, int32_t D, double T
   });
 }
 
+int cb_build_reactor(const cb_mechanism* m, int32_t lean, cb_program** out) {
+  return guard([&] {
+    require(m && out, "null argument");
+    *out = make_program(m->m, reactor(*m->m, lean != 0), ProgramKind::Other);
+  });
+}
+
+int cb_build_jacobian(const cb_program* p, int32_t n_wrt, const int32_t* wrt, int32_t n_rows, const int32_t* rows, cb_program** out) {
+  return guard([&] {
+    require(p && out && n_wrt >= 0 && n_rows >= 0, "bad argument");
+    require((n_wrt == 0 || wrt) && (n_rows == 0 || rows), "null index array");
+    ast::Linear l = ast::jacobian(p->linear, std::vector<int32_t>(wrt, wrt + n_wrt), std::vector<int32_t>(rows, rows + n_rows));
+    auto* j = new cb_program;
+    j->mechanism = p->mechanism;
+    j->function.n_input = l.n_input;
+    j->function.output.assign(l.output.size(), -1);  // the program lives in its linear form only
+    j->linear = std::move(l);
+    j->kind = ProgramKind::Other;
+    *out = j;
+  });
+}
+
 void cb_program_free(cb_program* p) { delete p; }
 
 int cb_program_sizes(const cb_program* p, int32_t* inputs, int32_t* outputs, int64_t* statements, int64_t* instructions, int32_t* registers) {
@@ -467,6 +500,63 @@ int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, s
 }
 
 void cb_module_free(cb_module* mod) { delete mod; }
+
+int cb_cpu_assemble(const cb_program* p, cb_cpu_module** out) {
+  return guard([&] {
+    require(p && out, "null argument");
+    auto* m = new cb_cpu_module;
+    try {
+      m->program = std::make_unique<HostProgram>(p->linear);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int cb_cpu_call(cb_cpu_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs, int32_t n_outputs,
+                int32_t threads) {
+  return guard([&] {
+    require(mod && mod->program, "null module");
+    require(n_inputs == mod->program->n_inputs() && n_outputs == mod->program->n_outputs(), "input/output count does not match the program");
+    require(n_states == 0 || (inputs && outputs), "null array table");
+    for (int32_t i = 0; i < n_inputs && n_states; i++) require(inputs[i], "null input array");
+    for (int32_t i = 0; i < n_outputs && n_states; i++) require(outputs[i], "null output array");
+    require(threads >= 1, "threads must be at least 1");
+    mod->program->call(inputs, n_states, outputs, threads);
+  });
+}
+
+void cb_cpu_module_free(cb_cpu_module* mod) { delete mod; }
+
+int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, cb_integrator** out) {
+  return guard([&] {
+    require(m, "null mechanism");
+    if (!out) return compile_integrator(m->m);
+    auto* it = new cb_integrator;
+    try {
+      it->integrator = std::make_unique<Integrator>(m->m, rtol, atol_T, atol_Y);
+    } catch (...) {
+      delete it;
+      throw;
+    }
+    *out = it;
+  });
+}
+
+int cb_integrator_advance(cb_integrator* it, size_t n_reactors, double* y, const double* P, double* t, double* h, int32_t* status,
+                          uint32_t* steps, double t_end, int32_t max_steps, void* stream, int64_t* steps_taken) {
+  return guard([&] {
+    require(it && it->integrator, "null integrator");
+    require(n_reactors == 0 || (y && P && t && h && status && steps), "null array");
+    require(max_steps >= 0, "max_steps must be non-negative");
+    const long taken = it->integrator->advance(y, P, t, h, status, steps, n_reactors, t_end, max_steps, (hipStream_t)stream);
+    if (steps_taken) *steps_taken = taken;
+  });
+}
+
+void cb_integrator_free(cb_integrator* it) { delete it; }
 
 int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w, cb_module** out) {
   return guard([&] {

@@ -21,7 +21,8 @@ struct StraightOptions {
   bool fast_transcendentals = false;
   // FP64 e^x inlined as a polynomial (exp64 in hip_emit.cpp, constants from exp_coefficients.py): the argument clamped
   // to [-746, 710] (outside it e^x rounds to 0 or overflows to +inf), Cody-Waite reduction to |r| <= ln(2)/2 with
-  // n taken from the low word of x log2(e) + 1.5·2^52, a degree-11 polynomial and v_ldexp_f64: no range branches.
+  // n taken from the low word of x log2(e) + 1.5·2^52, a degree-10 polynomial (relative error below 5e-16) and
+  // v_ldexp_f64: no range branches.
   bool exp_poly = false;
   // Registers kept in LDS instead of VGPRs (one FP64 slot per register and thread, [slot][thread] so that a wave's
   // accesses are consecutive): stored where defined and read where used. Every `fence_every` instructions a

@@ -105,7 +105,8 @@ class ExportModule : public SoAForm {
   }
 
  protected:
-  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
+  void launch(const double* const* in, double* const* out, size_t n, const Launch& how) override {
+    hipStream_t stream = how.stream;
     unsigned n32 = (unsigned)n, block = 256;
     void* args[] = {&in, &out, &n32};
     device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + block - 1) / block), 1, 1, block, 1, 1, 0, stream, args, nullptr),

@@ -25,7 +25,8 @@ class Straight : public SoAForm {
   }
 
  protected:
-  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
+  void launch(const double* const* in, double* const* out, size_t n, const Launch& how) override {
+    hipStream_t stream = how.stream;
     unsigned n32 = (unsigned)n, offset = 0;
     void* args[] = {&in, &out, &n32, &offset};
     unsigned grid = (unsigned)((n + block_size_ - 1) / block_size_);

@@ -46,6 +46,23 @@ class Form {
   virtual int n_outputs() const = 0;
   // inputs/outputs: tables of n_inputs()/n_outputs() arrays of n doubles, host or device memory.
   virtual void call(const double* const* inputs, size_t n, double* const* outputs, bool device, hipStream_t stream, float* kernel_ms) = 0;
+  // Device memory the form's kernels need beyond the arrays and their pointer table for n states (the surrogate's
+  // fallback counter and state list); 0 for the others.
+  virtual size_t work_bytes(size_t n) const {
+    (void)n;
+    return 0;
+  }
+  // The kernels alone, all on `stream`, with nothing allocated, created or synchronized: in/out are device tables of
+  // the arrays' device pointers and work is work_bytes(n) bytes of 256-byte aligned device memory (or null when 0).
+  virtual void launch_only(const double* const* in, double* const* out, size_t n, hipStream_t stream, void* work) = 0;
+};
+
+// How SoAForm::launch runs its kernels: on the caller's stream alone (single_stream) or with a forked second
+// stream of the module's; with the form's work memory given by the caller (work) or the module's own.
+struct Launch {
+  hipStream_t stream;
+  bool single_stream;
+  void* work;
 };
 
 // cb_hip_assemble's form argument: the form in the low bits, modifier flags above (cb_form, cb_form_flags).

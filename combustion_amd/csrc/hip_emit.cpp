@@ -49,17 +49,16 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
          "  const double n = t - 0x1.8p+52;\n"
          "  double r = __builtin_fma(-n, 0x1.62e42fee00000p-1, x);\n"
          "  r = __builtin_fma(-n, 0x1.a39ef35793c76p-33, r);\n"
-         "  double p = 0x1.af631d0059becp-26;\n"
-         "  p = __builtin_fma(p, r, 0x1.28b4057f44145p-22);\n"
-         "  p = __builtin_fma(p, r, 0x1.71ddf5749d126p-19);\n"
-         "  p = __builtin_fma(p, r, 0x1.a01991ac8730ap-16);\n"
-         "  p = __builtin_fma(p, r, 0x1.a01a01b14378fp-13);\n"
-         "  p = __builtin_fma(p, r, 0x1.6c16c187fbe02p-10);\n"
-         "  p = __builtin_fma(p, r, 0x1.111111110f225p-7);\n"
-         "  p = __builtin_fma(p, r, 0x1.555555554f0cfp-5);\n"
-         "  p = __builtin_fma(p, r, 0x1.555555555555ap-3);\n"
-         "  p = __builtin_fma(p, r, 0x1.0000000000011p-1);\n"
-         "  p = __builtin_fma(p, r, 1.0);\n"
+         "  double p = 0x1.28a2c0a7209fbp-22;\n"
+         "  p = __builtin_fma(p, r, 0x1.72faf024b693bp-19);\n"
+         "  p = __builtin_fma(p, r, 0x1.a019a66a75dd4p-16);\n"
+         "  p = __builtin_fma(p, r, 0x1.a01978c6baf81p-13);\n"
+         "  p = __builtin_fma(p, r, 0x1.6c16c17f43a58p-10);\n"
+         "  p = __builtin_fma(p, r, 0x1.1111112dd67c5p-7);\n"
+         "  p = __builtin_fma(p, r, 0x1.55555555520afp-5);\n"
+         "  p = __builtin_fma(p, r, 0x1.555555554b757p-3);\n"
+         "  p = __builtin_fma(p, r, 0x1.0000000000005p-1);\n"
+         "  p = __builtin_fma(p, r, 0x1.000000000001ep+0);\n"
          "  p = __builtin_fma(p, r, 1.0);\n"
          "  return __builtin_ldexp(p, (int)(unsigned)__double_as_longlong(t));\n"
          "}\n";

@@ -1,0 +1,151 @@
+// Batched constant-pressure reactors on the GPU: per step, the analytic Jacobian program (f and J in one kernel), the
+// three Rosenbrock phases of kernels/rosenbrock.hip and two right-hand-side evaluations, all on the caller's stream;
+// the batch is polled for reactors still running every few steps.
+#include "integrator.hpp"
+
+#include <sstream>
+
+#include "builders.hpp"
+#include "emit.hpp"
+#include "jacobian.hpp"
+
+namespace combustion {
+
+extern const char* const rosenbrock_source;
+
+namespace {
+constexpr int kPollEvery = 8;
+}
+
+// The right-hand side program, its Jacobian program (d [dT/dt, dY/dt] / d [T, Y]; P is a parameter) and the
+// Rosenbrock kernels' source, for a mechanism.
+struct IntegratorPrograms {
+  int m;
+  ast::Function rhs_function, jacobian_function;
+  ast::Linear rhs_linear, jacobian_linear;
+  std::string step_source;
+  explicit IntegratorPrograms(const Mechanism& mech) : m((int)mech.species.len() + 1) {
+    if (m > 64) throw Unsupported("the batched integrator holds a reactor's state in one wave: at most 63 species");
+    rhs_function = reactor(mech, false);
+    rhs_linear = ast::linearize(rhs_function);
+    std::vector<int32_t> wrt = {0}, rows;
+    for (int k = 0; k + 1 < m; k++) wrt.push_back(2 + k);
+    for (int o = 0; o < m; o++) rows.push_back(o);
+    jacobian_linear = ast::jacobian(rhs_linear, wrt, rows);
+    jacobian_function.n_input = jacobian_linear.n_input;
+    jacobian_function.output.assign(jacobian_linear.output.size(), -1);
+    std::ostringstream src;
+    src << "#define M " << m << "\n" << rosenbrock_source;
+    step_source = src.str();
+  }
+};
+
+void compile_integrator(std::shared_ptr<const Mechanism> m) {
+  IntegratorPrograms p(*m);
+  compile_form(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, 1, 0);
+  compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, 0);
+  device::compile(p.step_source, "rosenbrock", hip_options(true));
+}
+
+Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double atol_T, double atol_Y)
+    : mech_(std::move(m)), m_((int)mech_->species.len() + 1), rtol_(rtol), atol_T_(atol_T), atol_Y_(atol_Y) {
+  if (!(rtol > 0) || !(atol_T > 0) || !(atol_Y > 0)) throw ArgumentError("tolerances must be positive");
+  IntegratorPrograms p(*mech_);
+  rhs_function_ = std::move(p.rhs_function);
+  rhs_linear_ = std::move(p.rhs_linear);
+  jacobian_function_ = std::move(p.jacobian_function);
+  jacobian_linear_ = std::move(p.jacobian_linear);
+  rhs_ = assemble_form(ProgramView{mech_, &rhs_function_, &rhs_linear_, ProgramKind::Other, nullptr}, 1, 0);
+  jacobian_ = assemble_form(ProgramView{mech_, &jacobian_function_, &jacobian_linear_, ProgramKind::Other, nullptr}, 1, 0);
+  step_.load(device::compile(p.step_source, "rosenbrock", hip_options(true)), "rosenbrock");
+}
+
+Integrator::~Integrator() { release(); }
+
+void Integrator::release() {
+  for (void* p : buffers_) (void)hipFree(p);
+  buffers_.clear();
+  n_ = 0;
+  last_y_ = nullptr;
+  last_P_ = nullptr;
+}
+
+void Integrator::prepare(double* y, const double* P, size_t n) {
+  if (n == n_ && y == last_y_ && P == last_P_) return;
+  device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  release();
+  const size_t m = (size_t)m_;
+  auto alloc = [&](size_t bytes) {
+    void* p = nullptr;
+    device::check(hipMalloc(&p, bytes), "hipMalloc integrator");
+    buffers_.push_back(p);
+    return p;
+  };
+  jac_ = (double*)alloc((m + m * m) * n * sizeof(double));
+  F_ = (double*)alloc(m * n * sizeof(double));
+  ys_ = (double*)alloc(m * n * sizeof(double));
+  k1_ = (double*)alloc(m * n * sizeof(double));
+  k2_ = (double*)alloc(m * n * sizeof(double));
+  F1_ = (double*)alloc(m * n * sizeof(double));
+  lu_ = (double*)alloc(m * m * n * sizeof(double));
+  perm_ = (int*)alloc(m * n * sizeof(int));
+  running_ = (unsigned*)alloc(256);
+  // pointer tables of the two programs: inputs [T, P, Y_0..Y_{K-1}] from y or ys; outputs F rows or jac rows
+  auto table = [&](const std::vector<const void*>& ptrs) {
+    void** t = (void**)alloc(ptrs.size() * sizeof(void*));
+    device::check(hipMemcpy(t, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice), "hipMemcpy table");
+    return t;
+  };
+  auto inputs = [&](const double* base) {
+    std::vector<const void*> p = {base, P};
+    for (size_t i = 1; i < m; i++) p.push_back(base + i * n);
+    return p;
+  };
+  auto rows = [&](const double* base, size_t count) {
+    std::vector<const void*> p;
+    for (size_t i = 0; i < count; i++) p.push_back(base + i * n);
+    return p;
+  };
+  tab_y_ = table(inputs(y));
+  tab_ys_ = table(inputs(ys_));
+  tab_jac_ = table(rows(jac_, m + m * m));
+  tab_F_ = table(rows(F_, m));
+  n_ = n;
+  last_y_ = y;
+  last_P_ = P;
+}
+
+long Integrator::advance(double* y, const double* P, double* t, double* h, int* status, unsigned* steps, size_t n, double t_end,
+                         int max_steps, hipStream_t stream) {
+  if (n == 0) return 0;
+  if (n >= (size_t(1) << 31)) throw ArgumentError("at most 2^31 - 1 reactors");
+  prepare(y, P, n);
+  hipFunction_t running = nullptr, clear = nullptr;
+  device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
+  device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
+  unsigned n32 = (unsigned)n;
+  double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
+  long taken = 0;
+  for (int step = 0; step < max_steps; step++) {
+    jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
+    for (int phase = 1; phase <= 3; phase++) {
+      void* args[] = {&phase, &n32, &jac_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
+      device::check(hipModuleLaunchKernel(step_.function, n32, 1, 1, 64, 1, 1, 0, stream, args, nullptr), "launch rosenbrock");
+      if (phase < 3) rhs_->launch_only((const double* const*)tab_ys_, (double* const*)tab_F_, n, stream, nullptr);
+    }
+    taken++;
+    if ((step + 1) % kPollEvery == 0 || step + 1 == max_steps) {
+      void* cargs[] = {&running_};
+      device::check(hipModuleLaunchKernel(clear, 1, 1, 1, 64, 1, 1, 0, stream, cargs, nullptr), "launch rosenbrock_clear");
+      void* rargs[] = {&n32, &status, &running_};
+      device::check(hipModuleLaunchKernel(running, (n32 + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr), "launch rosenbrock_running");
+      unsigned left = 0;
+      device::check(hipMemcpyAsync(&left, running_, sizeof left, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+      device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (left == 0) break;
+    }
+  }
+  return taken;
+}
+
+}  // namespace combustion

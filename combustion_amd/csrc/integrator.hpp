@@ -1,0 +1,54 @@
+// Batched stiff integration of constant-pressure reactors on the GPU (the role of cvode/lib.rs:17-47 and the
+// integration loop of plot.rs:80-118): the reactor right-hand side and its analytic Jacobian as FP64 straight-line
+// kernels, and the Rosenbrock stages in kernels/rosenbrock.hip.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <vector>
+
+#include "ast.hpp"
+#include "device.hpp"
+#include "forms.hpp"
+#include "mechanism.hpp"
+
+namespace combustion {
+
+// Fills the code-object cache with the integrator's kernels for a mechanism (no GPU needed).
+void compile_integrator(std::shared_ptr<const Mechanism> m);
+
+class Integrator {
+ public:
+  // Tolerances as CVODE's scalar pair (cvode/lib.rs:30): rtol relative; atol_T for the temperature and atol_Y for
+  // the mass fractions.
+  Integrator(std::shared_ptr<const Mechanism> m, double rtol, double atol_T, double atol_Y);
+  ~Integrator();
+  int state_size() const { return m_; }
+  // Advances n reactors to t_end in place, all arrays device memory: y [K+1][n] (T, then Y_0..Y_{K-1}, SoA), P [n],
+  // t [n], h [n] (each reactor's time and next step size), status [n] (0 running, 1 at t_end, 2 step size
+  // collapsed; reactors not at 0 are left alone), steps [n] (incremented per attempted step). At most max_steps
+  // steps; returns the steps taken by the batch.
+  long advance(double* y, const double* P, double* t, double* h, int* status, unsigned* steps, size_t n, double t_end, int max_steps,
+               hipStream_t stream);
+
+ private:
+  std::shared_ptr<const Mechanism> mech_;
+  int m_;
+  double rtol_, atol_T_, atol_Y_;
+  ast::Function rhs_function_, jacobian_function_;
+  ast::Linear rhs_linear_, jacobian_linear_;
+  std::unique_ptr<Form> rhs_, jacobian_;
+  device::Kernel step_;
+  size_t n_ = 0;
+  std::vector<void*> buffers_;
+  double *jac_ = nullptr, *F_ = nullptr, *ys_ = nullptr, *k1_ = nullptr, *k2_ = nullptr, *F1_ = nullptr, *lu_ = nullptr;
+  int* perm_ = nullptr;
+  unsigned* running_ = nullptr;
+  void **tab_y_ = nullptr, **tab_ys_ = nullptr, **tab_jac_ = nullptr, **tab_F_ = nullptr;
+  const double* last_y_ = nullptr;
+  const double* last_P_ = nullptr;
+  void release();
+  void prepare(double* y, const double* P, size_t n);
+};
+
+}  // namespace combustion

@@ -361,6 +361,16 @@ class Reactions {
     if (std::isnan(n.temperature_split)) {
       auto a = gibbs0_RT(n.pieces[0]);
       e = dot(f, consts(f, a.data(), 7), gibbs_basis(f, T_));
+    } else if (lean_) {
+      // Lean form: both pieces evaluated, then the value selected (the same value as selecting the coefficients
+      // first, reaction.rs:85-107): each piece's constants enter its FMAs as scalar operands, where selecting seven
+      // coefficient pairs per species costs a VGPR move and a select per 32-bit half.
+      auto a = gibbs0_RT(n.pieces[0]), b = gibbs0_RT(n.pieces[1]);
+      const std::vector<E> basis = gibbs_basis(f, T_);
+      E low = dot(f, consts(f, a.data(), 7), basis), high = dot(f, consts(f, b.data(), 7), basis);
+      const int32_t id = f.value("Gibbs0_RT" + std::to_string(k));
+      f.select(f.le(T_.T, f.c(n.temperature_split)), {low}, {high}, {id});
+      return g_[k] = f.v(id);
     } else {
       auto a = gibbs0_RT(n.pieces[0]), b = gibbs0_RT(n.pieces[1]);
       std::vector<E> t, fe, piece;
@@ -378,12 +388,27 @@ class Reactions {
     return g_[k] = f.defv(e, "Gibbs0_RT" + std::to_string(k));
   }
   E exp(size_t k) {
-    if (exp_[k] < 0) exp_[k] = f_.defv(f_.min(f_.c(exp_clamp), f_.exp(gibbs(k))), "expGibbs0_RT" + std::to_string(k));
+    if (exp_[k] < 0) {
+      if (lean_) lean_exponentials(k);
+      else exp_[k] = f_.defv(f_.min(f_.c(exp_clamp), f_.exp(gibbs(k))), "expGibbs0_RT" + std::to_string(k));
+    }
     return exp_[k];
   }
   E exp_neg(size_t k) {
-    if (exp_neg_[k] < 0) exp_neg_[k] = f_.defv(f_.min(f_.c(exp_neg_clamp), f_.exp(f_.neg(gibbs(k)))), "exp_negGibbs0_RT" + std::to_string(k));
+    if (exp_neg_[k] < 0) {
+      if (lean_) lean_exponentials(k);
+      else exp_neg_[k] = f_.defv(f_.min(f_.c(exp_neg_clamp), f_.exp(f_.neg(gibbs(k)))), "exp_negGibbs0_RT" + std::to_string(k));
+    }
     return exp_neg_[k];
+  }
+  // Lean form: one exponential per species, G = exp(G_k/RT), and exp(-G_k/RT) as 1/G (a reciprocal instead of a
+  // second exponential; the same value to a few ulps, and the same clamped results where either side overflows).
+  // Both clamped values are defined together, so that G is dead right after them.
+  void lean_exponentials(size_t k) {
+    Block& f = f_;
+    E G = f.defv(f.exp(gibbs(k)), "expGibbs0_RT_unclamped" + std::to_string(k));
+    exp_[k] = f.defv(f.min(f.c(exp_clamp), G), "expGibbs0_RT" + std::to_string(k));
+    exp_neg_[k] = f.defv(f.min(f.c(exp_neg_clamp), f.div(f.c(1.), G)), "exp_negGibbs0_RT" + std::to_string(k));
   }
 };
 
@@ -552,6 +577,33 @@ ast::Function chemistry(const Mechanism& m, const Polynomials* poly, const Trans
     (void)dtE;
     out.push_back(f.mul(T, dtT_T));
   }
+  return f.finish(out);
+}
+
+// The constant-pressure reactor's right-hand side in the variables a batch integrator steps (the role of the
+// reference's cvode/lib.rs:17-47 right-hand side and plot.rs:80-118): inputs [T, P, Y_0..Y_{K-1}], outputs
+// [dT/dt, dY_0/dt..dY_{K-1}/dt] with dT/dt the energy balance of reaction.rs:375-380 and dY_k/dt = W_k ω_k / ρ,
+// ρ = P / (R T Σ_j Y_j/W_j) (zero for the species that do not react).
+ast::Function reactor(const Mechanism& m, bool lean) {
+  const size_t K = m.species.len();
+  if (K < 2) throw ast::Error("need at least two species");
+  Block f(names({"T", "P"}, K, "Y"));
+  E T = f.v(0), P = f.v(1);
+  std::vector<E> n;
+  for (size_t k = 0; k < K; k++) n.push_back(f.defv(f.mul(f.v(2 + (int32_t)k), f.c(1. / m.species.molar_mass[k])), "amounts"));
+  E pressure_R = f.defv(f.mul(P, f.c(1. / R)), "pressure_R");
+  E sum_n = f.defv(f.sum(n), "Σamounts");
+  E rcpT0 = f.defv(f.div(f.c(1.), T), "rcpT");
+  E molar_density = f.defv(f.div(f.mul(pressure_R, rcpT0), sum_n), "molar_density");
+  std::vector<E> nonbulk(n.begin(), n.end() - 1);
+  Prologue p = prologue(f, T, pressure_R, molar_density, nonbulk);
+  std::vector<E> omega = species_rates(f, m, p.T, p.concentration, p.C, lean);
+  auto [dtT_T, dtE] = energy_terms(f, m, p.T, p.C, omega);
+  (void)dtE;
+  E rcp_rho = f.defv(f.div(f.mul(T, sum_n), pressure_R), "1/ρ");
+  std::vector<E> out = {f.mul(T, dtT_T)};
+  for (size_t k = 0; k < K; k++)
+    out.push_back(k < m.active ? f.mul(f.mul(f.c(m.species.molar_mass[k]), omega[k]), rcp_rho) : f.c(0.));
   return f.finish(out);
 }
 

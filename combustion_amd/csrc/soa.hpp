@@ -52,8 +52,13 @@ class SoAForm : public Form {
   }
 
  protected:
+  void launch_only(const double* const* in, double* const* out, size_t n, hipStream_t stream, void* work) override {
+    if (n) launch(in, out, n, Launch{stream, true, work});
+  }
+
+ protected:
   // in/out: device tables of device pointers.
-  virtual void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) = 0;
+  virtual void launch(const double* const* in, double* const* out, size_t n, const Launch& how) = 0;
 
   // Orders this call after the module's previous call when the two come on different streams, so that state the
   // module keeps between its kernels (the surrogate's fallback list) is never shared by two calls in flight. Not
@@ -129,13 +134,13 @@ class SoAForm : public Form {
         device::check(hipEventCreate(&stop_), "hipEventCreate");
       }
       device::check(hipEventRecord(start_, stream), "hipEventRecord");
-      launch(in, out, n, stream);
+      launch(in, out, n, Launch{stream, false, nullptr});
       device::check(hipEventRecord(stop_, stream), "hipEventRecord");
       mark_call_done(stream);
       device::check(hipEventSynchronize(stop_), "hipEventSynchronize");
       device::check(hipEventElapsedTime(kernel_ms, start_, stop_), "hipEventElapsedTime");
     } else {
-      launch(in, out, n, stream);
+      launch(in, out, n, Launch{stream, false, nullptr});
       mark_call_done(stream);
     }
   }

@@ -349,7 +349,9 @@ std::string rates_straight_text(const ProgramView& p, bool fast) {
   // AGPRs and scratch; its clamped exp(-G_k/RT) values then live in LDS (K_a slots of 2 KB per block of 256). At
   // K=53 (about 230 live doubles with the lean rates form) the kernel then needs no scratch and takes 1.25 ms per 2^20
   // states against 1.30 ms with the values in registers; keeping some exp(+G_k/RT) values there too, a fence every
-  // 64 or 256 instructions, or two waves per SIMD with 40 slots (1.6 ms, 1.2 KB of scratch) were not faster.
+  // 64 or 256 instructions, or two waves per SIMD with 40 slots (1.6 ms, 1.2 KB of scratch) were not faster, nor
+  // was reading the FP64 constants from a __constant__ table by scalar loads (3.2 k s_mov per wave fall to 1.8 k
+  // SALU instructions, but 1.48 ms against 1.18 ms: one wave per SIMD leaves the loads' latency exposed).
   // Beyond 80 slots the values stay in registers.
   if (ast::peak_live(l) > 128) so.lds_registers = gibbs_exponentials(l, true);
   if (so.lds_registers.size() * so.block_size * 8 > 160 * 1024) so.lds_registers.clear();
@@ -489,8 +491,9 @@ class Tiled : public SoAForm {
   // The rates and transport kernels read the same inputs and write disjoint outputs: the transport kernel runs on a
   // second stream of the module's (default priority), forked from and joined back into the caller's stream, so that
   // the two kernels' blocks share the CUs at the ends of each launch (2.27 against 2.32 ms per 2^20 states at K=53).
-  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
-    const bool both = (mixed_ || fp64_) && (straight_ || transport_);
+  void launch(const double* const* in, double* const* out, size_t n, const Launch& how) override {
+    hipStream_t stream = how.stream;
+    const bool both = (mixed_ || fp64_) && (straight_ || transport_) && !how.single_stream;
     hipStream_t ts = stream;
     if (both) {
       if (!side_) {
@@ -642,19 +645,29 @@ class Surrogate : public SoAForm {
   }
 
  protected:
-  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) override {
-    if (n > capacity_) {
+  // Work memory: the fallback counter (one unsigned in the first 256 bytes), then the list of fallback states.
+  size_t work_bytes(size_t n) const override { return 256 + n * sizeof(unsigned); }
+
+  void launch(const double* const* in, double* const* out, size_t n, const Launch& how) override {
+    hipStream_t stream = how.stream;
+    unsigned *count = d_count_, *list = d_fallback_;
+    if (how.work) {
+      count = (unsigned*)how.work;
+      list = (unsigned*)((char*)how.work + 256);
+    } else if (n > capacity_) {
       // an earlier call on another stream may still fill the list
       device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       if (d_fallback_) device::check(hipFree(d_fallback_), "hipFree");
       d_fallback_ = nullptr;
       device::check(hipMalloc(&d_fallback_, n * sizeof(unsigned)), "hipMalloc");
       capacity_ = n;
+      list = d_fallback_;
     }
     // The rates kernel on the caller's stream; the surrogate and its fallback on a side stream forked from and joined
     // back into it (the two read the same inputs and write disjoint outputs), as in the tiled forms.
     hipStream_t ts = stream;
-    if (rates_) {
+    const bool fork = rates_ && !how.single_stream;
+    if (fork) {
       if (!side_) {
         device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
         device::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
@@ -663,21 +676,21 @@ class Surrogate : public SoAForm {
       device::check(hipEventRecord(fork_, stream), "hipEventRecord");
       device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
       ts = side_;
-      rates_->launch(in, out, n, stream);
     }
+    if (rates_) rates_->launch(in, out, n, stream);
     double* const* tout = out + n_rates_;
-    void* clear_args[] = {&d_count_};
+    void* clear_args[] = {&count};
     device::check(hipModuleLaunchKernel(clear_, 1, 1, 1, 64, 1, 1, 0, ts, clear_args, nullptr), "launch surrogate_clear");
     unsigned n32 = (unsigned)n;
     void* args[] = {&in, &tout, &n32, &d_w16_, &d_params_, &d_offsets_, &d_f_mean_, &d_f_rstd_, &d_lo_, &d_hi_,
                     &d_o_mean_, &d_o_std_, &d_bc_, &d_bv_, &d_major_, &d_slot_, &d_wa_, &d_wb_, &d_bd_, &d_rw_, &T0_,
-                    &d_fallback_, &d_count_};
+                    &list, &count};
     const unsigned per_block = 64;
     device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + per_block - 1) / per_block), 1, 1, 256, 1, 1, 0, ts, args,
                                         nullptr),
                   "launch surrogate_transport");
-    transport_->launch(in, tout, n, ts, d_fallback_, d_count_);
-    if (rates_) {
+    transport_->launch(in, tout, n, ts, list, count);
+    if (fork) {
       device::check(hipEventRecord(join_, side_), "hipEventRecord");
       device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
     }

@@ -69,6 +69,8 @@ SIGNATURES = {
     "cb_build_reactions_rates": (C.c_int, [_P, C.POINTER(_P)]),
     "cb_build_transport": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
     "cb_build_chemistry": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
+    "cb_build_reactor": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "cb_build_jacobian": (C.c_int, [_P, C.c_int32, _I32P, C.c_int32, _I32P, C.POINTER(_P)]),
     "cb_program_free": (None, [_P]),
     "cb_program_sizes": (C.c_int, [_P, _I32P, _I32P, _I64P, _I64P, _I32P]),
     "cb_program_op_counts": (C.c_int, [_P, _I64P]),
@@ -82,6 +84,12 @@ SIGNATURES = {
     "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
     "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
     "cb_module_free": (None, [_P]),
+    "cb_integrator_create": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
+    "cb_integrator_advance": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int32, _P, _I64P]),
+    "cb_integrator_free": (None, [_P]),
+    "cb_cpu_assemble": (C.c_int, [_P, C.POINTER(_P)]),
+    "cb_cpu_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32]),
+    "cb_cpu_module_free": (None, [_P]),
     "cb_hip_assemble_surrogate": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "cb_module_fallback_count": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
     "cb_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_P)]),
@@ -238,6 +246,12 @@ class Mechanism:
         _check(lib().cb_hip_assemble_export(self._h, D, C.byref(h)))
         return Module(h, _Signature(self.K + 2, self.active + 2 + self.K))
 
+    def reactor(self, lean=False):
+        """cb_build_reactor: [T, P, Y_0..Y_{K-1}] -> [dT/dt, dY_0/dt..dY_{K-1}/dt] at constant pressure."""
+        h = _P()
+        _check(lib().cb_build_reactor(self._h, int(lean), C.byref(h)))
+        return Program(h, self)
+
     def chemistry(self, flags=RATES | TRANSPORT, D=4, T0=1000.0, viscosity=0.0, conductivity=0.0):
         """Per-state SoA program: inputs [T, P, Y_0..Y_{K-1}] -> [ω_0..ω_{Ka-1}, λ, μ, D_0..D_{K-1}, (dT/dt)]."""
         h = _P()
@@ -296,12 +310,28 @@ class Program:
                                             C.byref(h)))
         return cls(h, None)
 
+    def jacobian(self, wrt, rows=None):
+        """cb_build_jacobian: outputs [own outputs, d output[rows[i]] / d input[wrt[j]] row-major]; rows default to all
+        outputs."""
+        wrt = np.ascontiguousarray(wrt, np.int32)
+        rows = np.ascontiguousarray(range(self.n_outputs) if rows is None else rows, np.int32)
+        h = _P()
+        _check(lib().cb_build_jacobian(self._h, len(wrt), wrt.ctypes.data_as(_I32P), len(rows), rows.ctypes.data_as(_I32P), C.byref(h)))
+        return Program(h, self.mechanism)
+
     def hip_source(self, form="straight", block_size=0):
         return _text(lib().cb_program_hip_source, self._h, form_code(form), block_size)
 
     def compile(self, form="straight", block_size=0):
         """Fills the code-object cache for a form (hiprtc for gfx950, no GPU needed)."""
         _check(lib().cb_hip_compile(self._h, form_code(form), block_size))
+
+    def assemble_cpu(self):
+        """ir::compile + ir::assemble (ir/lib.rs:225-263) with the loop of device.rs:28-48: the program as C compiled
+        by the host compiler and called once per state (cb_cpu_assemble)."""
+        h = _P()
+        _check(lib().cb_cpu_assemble(self._h, C.byref(h)))
+        return CpuModule(h, self)
 
     def assemble(self, block_size=0, form="straight"):
         """device::assemble (device.rs:25): emit, compile with hiprtc, load the module."""
@@ -371,6 +401,71 @@ class Module:
                                  C.byref(ms) if time_kernel else None))
         if time_kernel:
             self.last_kernel_ms = ms.value
+        return outputs
+
+
+class Integrator:
+    """cb_integrator_*: batched constant-pressure reactors on the GPU (Rosenbrock ode23s with the analytic Jacobian).
+    advance(y, P, t, h, status, steps, t_end) on contiguous torch device tensors: y float64 [K+1][n] (T, Y_k), P, t, h
+    float64 [n], status int32 [n], steps int32 [n]; returns the batch's step count."""
+
+    def __init__(self, mechanism, rtol=1e-6, atol_T=1e-6, atol_Y=1e-12):
+        h = _P()
+        _check(lib().cb_integrator_create(mechanism._h, rtol, atol_T, atol_Y, C.byref(h)))
+        self._h = h
+        self.mechanism = mechanism
+
+    @staticmethod
+    def compile(mechanism):
+        _check(lib().cb_integrator_create(mechanism._h, 1e-6, 1e-6, 1e-12, None))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cb_integrator_free(self._h)
+            self._h = None
+
+    def advance(self, y, P, t, h, status, steps, t_end, max_steps=100000, stream=None):
+        n = P.shape[0]
+        m = self.mechanism.K + 1
+        if tuple(y.shape) != (m, n):
+            raise ValueError(f"y must be [{m}][{n}]")
+        for x, dtype in ((y, "torch.float64"), (P, "torch.float64"), (t, "torch.float64"), (h, "torch.float64"), (status, "torch.int32"),
+                         (steps, "torch.int32")):
+            if not getattr(x, "is_cuda", False) or str(x.dtype) != dtype or not x.is_contiguous() or x.shape[-1] != n:
+                raise ValueError("integrator arrays must be contiguous device tensors of the documented dtypes")
+        taken = C.c_int64(0)
+        _check(lib().cb_integrator_advance(self._h, n, y.data_ptr(), P.data_ptr(), t.data_ptr(), h.data_ptr(), status.data_ptr(),
+                                           steps.data_ptr(), float(t_end), int(max_steps), stream, C.byref(taken)))
+        return taken.value
+
+
+class CpuModule:
+    """cb_cpu_call: module(inputs, outputs=None, threads=1) over SoA host arrays (numpy float64)."""
+
+    def __init__(self, handle, program):
+        self._h = handle
+        self.program = program
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cb_cpu_module_free(self._h)
+            self._h = None
+
+    def __call__(self, inputs, outputs=None, threads=1):
+        if len(inputs) != self.program.n_inputs:
+            raise ValueError(f"expected {self.program.n_inputs} inputs, got {len(inputs)}")
+        inputs = [np.ascontiguousarray(x, np.float64) for x in inputs]
+        n = len(inputs[0])
+        if any(len(x) != n for x in inputs):
+            raise ValueError("all inputs must have the same length")
+        if outputs is None:
+            outputs = [np.empty(n) for _ in range(self.program.n_outputs)]
+        for x in outputs:
+            if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous and len(x) == n):
+                raise ValueError("outputs must be contiguous float64 arrays of the inputs' length")
+        in_arr = (C.c_void_p * len(inputs))(*[x.ctypes.data for x in inputs])
+        out_arr = (C.c_void_p * len(outputs))(*[x.ctypes.data for x in outputs])
+        _check(lib().cb_cpu_call(self._h, in_arr, len(inputs), n, out_arr, len(outputs), int(threads)))
         return outputs
 
 

@@ -24,6 +24,8 @@ extern "C" {
 typedef struct cb_mechanism cb_mechanism;
 typedef struct cb_program cb_program;
 typedef struct cb_module cb_module;
+typedef struct cb_cpu_module cb_cpu_module;
+typedef struct cb_integrator cb_integrator;
 
 enum cb_status {
   CB_OK = 0,
@@ -95,6 +97,16 @@ int cb_build_transport(const cb_mechanism* m, int32_t D, double T0, double visco
  * (K, Pa, mass fractions); outputs [ω_0..ω_{Ka-1}] if CB_RATES, [λ, μ, D_0..D_{K-1}] if CB_TRANSPORT,
  * [dT/dt] if CB_RHS, in that order. viscosity/conductivity <= 0 select cb_transport_scales. */
 int cb_build_chemistry(const cb_mechanism* m, int32_t flags, int32_t D, double T0, double viscosity, double conductivity, cb_program** out);
+/* The constant-pressure reactor's right-hand side in the variables a batch integrator steps (the role of the right-hand
+ * side of cvode/lib.rs:17-47 and plot.rs:80-118): inputs [T, P, Y_0..Y_{K-1}], outputs [dT/dt, dY_0/dt..dY_{K-1}/dt],
+ * dT/dt the energy balance of reaction.rs:375-380 and dY_k/dt = W_k ω_k / ρ with ρ = P / (R T Σ_j Y_j/W_j) (zero for
+ * the species that do not react). lean != 0 takes the shapes of CB_LEAN. */
+int cb_build_reactor(const cb_mechanism* m, int32_t lean, cb_program** out);
+/* Analytic Jacobian by forward-mode differentiation of a program (no counterpart in the reference, whose integrator
+ * takes CVODE's finite-difference Jacobians): a program with p's inputs whose outputs are p's outputs followed by
+ * d output[rows[i]] / d input[wrt[j]], row-major (n_rows x n_wrt). min and max differentiate the operand they select
+ * (the first on ties), a select its chosen operand. Assemble it like any other program. */
+int cb_build_jacobian(const cb_program* p, int32_t n_wrt, const int32_t* wrt, int32_t n_rows, const int32_t* rows, cb_program** out);
 void cb_program_free(cb_program* p);
 int cb_program_sizes(const cb_program* p, int32_t* inputs, int32_t* outputs, int64_t* statements, int64_t* instructions, int32_t* registers);
 /* cuda.rs:229-254 categories over the linear program: alu, div, sqrt, exp, ln, neg, select, constants. */
@@ -141,6 +153,33 @@ int cb_hip_compile(const cb_program* p, int32_t form, uint32_t block_size);
 int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs, int32_t n_outputs,
                 int32_t pointers_are_device, void* stream, float* kernel_ms);
 void cb_module_free(cb_module* mod);
+
+/* The CPU path: ir/lib.rs:225-263 compile + assemble (the reference's Cranelift JIT of one per-state function,
+ * `extern fn(input: *const T, output: *mut T)`) and the loop of device.rs:28-48 around it. The program is emitted as
+ * straight-line C in FP64 (libm exp/log, no FP contraction), compiled by the host compiler ($COMBUSTION_AMD_HOST_CC,
+ * else ROCm's clang) at -O2 into a cached shared object and dlopen'ed. cb_cpu_call gathers each state's inputs
+ * (SoA host arrays of n_states doubles) into one AoS buffer, calls the function and scatters its outputs, on
+ * `threads` host threads over contiguous ranges of states. The benchmark's CPU baseline; the HIP entry points never
+ * fall back to it. */
+int cb_cpu_assemble(const cb_program* p, cb_cpu_module** out);
+int cb_cpu_call(cb_cpu_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs, int32_t n_outputs,
+                int32_t threads);
+void cb_cpu_module_free(cb_cpu_module* mod);
+
+/* Batched stiff integration of constant-pressure reactors on the GPU, the counterpart of the reference's CVODE
+ * wrapper (cvode/lib.rs:17-47: CVodeCreate(CV_BDF), CVodeSStolerances(rtol, atol), dense linear solver, one step at
+ * a time) and of the reactor loop of plot.rs:80-118, for many reactors at once: the L-stable (2,3) Rosenbrock pair
+ * of ode23s (Shampine & Reichelt 1997) with the analytic Jacobian of cb_build_reactor's program (cb_build_jacobian)
+ * and one wave per reactor for the LU factorisation and the solves (state size K+1 <= 64). Tolerances: rtol, and
+ * atol_T (temperature) / atol_Y (mass fractions). With out == NULL the kernels are only compiled into the cache.
+ * cb_integrator_advance moves n reactors toward t_end, in place, every array in device memory: y [K+1][n] (SoA rows
+ * T, Y_0..Y_{K-1}), P [n] (constant), t [n] and h [n] (each reactor's time and next step size), status [n] (0
+ * running; set to 1 at t_end, 2 when the step size collapsed; others are left alone), steps [n] (attempted steps).
+ * It stops after max_steps steps of the batch or when no reactor runs, and blocks until then. */
+int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, cb_integrator** out);
+int cb_integrator_advance(cb_integrator* it, size_t n_reactors, double* y, const double* P, double* t, double* h, int32_t* status,
+                          uint32_t* steps, double t_end, int32_t max_steps, void* stream, int64_t* steps_taken);
+void cb_integrator_free(cb_integrator* it);
 
 /* Transport surrogate network (no counterpart in the reference, whose transport is always evaluated by the
  * transport.rs formulas; this is the learned replacement of that section). An MLP n_in -> width (SiLU) x depth ->

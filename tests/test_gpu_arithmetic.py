@@ -1,5 +1,5 @@
 """The FP64 building blocks that the FP64 chemistry kernels inline instead of the device library's: the polynomial
-exponential (Cody-Waite reduction, degree-11 polynomial, v_ldexp_f64) and the reciprocal by v_rcp_f64 and one Newton
+exponential (Cody-Waite reduction, degree-10 polynomial, v_ldexp_f64) and the reciprocal by v_rcp_f64 and one Newton
 step. Each runs as a one-instruction program in form 1 (the straight-line form that uses them) over dense and edge
 inputs, against numpy's correctly rounded (to within an ulp) results: the bounds written here are what the
 chemistry kernels' 1e-12 parity relies on, with three orders of magnitude to spare."""
@@ -33,7 +33,7 @@ def test_polynomial_exponential(gpu):
     normal = (ref >= np.finfo(np.float64).tiny) & np.isfinite(ref)
     rel = np.abs(got[normal] - ref[normal]) / ref[normal]
     print("exp64 max relative error (normal results)", rel.max(), flush=True)
-    assert rel.max() < 4.5e-16  # two ulps
+    assert rel.max() < 6.7e-16  # three ulps: the polynomial's 4.3e-16 and the reduction's and Horner steps' rounding
     sub = ref < np.finfo(np.float64).tiny  # denormal and zero results: within two units of the smallest denormal
     assert np.all(np.abs(got[sub] - ref[sub]) <= 2 * 2.0 ** -1074)
     assert np.all(got[x >= 709.79] == np.inf) and np.all(got[x <= -746.] == 0.)

@@ -1,0 +1,104 @@
+"""Analytic Jacobians (cb_build_jacobian, forward-mode differentiation of a program) on the CPU through the FP64
+interpreter: each operation's derivative in closed form, and the constant-pressure reactor's Jacobian against central
+differences of the same program. The reactor right-hand side itself (cb_build_reactor) against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from combustion_amd import synth
+from combustion_amd.native import Program
+from helpers import setup
+
+OP = {"const": 0, "neg": 1, "min": 2, "max": 3, "add": 4, "sub": 5, "le": 6, "mul": 7, "div": 8, "sqrt": 9, "exp": 10, "ln": 11,
+      "sq": 12, "select": 13}
+R = 1.380649e-23 * 6.02214076e23
+
+
+def program(ops, n_inputs, outputs):
+    """A linear program from (op, a, b, c, k) tuples defining registers n_inputs, n_inputs+1, ..."""
+    lin = {"op": [], "dst": [], "a": [], "b": [], "c": [], "k": [], "outputs": outputs, "n_inputs": n_inputs}
+    for i, (op, a, b, c, k) in enumerate(ops):
+        for key, v in zip(("op", "dst", "a", "b", "c", "k"), (OP[op], n_inputs + i, a, b, c, k)):
+            lin[key].append(v)
+    return Program.from_linear(lin)
+
+
+def test_each_operation_in_closed_form():
+    # inputs x (0), y (1); registers 2..: x*y, exp(x), x/y, min(x,y), max(x,y), sqrt(x), ln(x), x*x, (x<=y), sel, -x, x-y
+    ops = [("mul", 0, 1, 0, 0), ("exp", 0, 0, 0, 0), ("div", 0, 1, 0, 0), ("min", 0, 1, 0, 0), ("max", 0, 1, 0, 0),
+           ("sqrt", 0, 0, 0, 0), ("ln", 0, 0, 0, 0), ("sq", 0, 0, 0, 0), ("le", 0, 1, 0, 0), ("select", 10, 2, 3, 0),
+           ("neg", 0, 0, 0, 0), ("sub", 0, 1, 0, 0), ("const", 0, 0, 0, 3.5), ("mul", 14, 0, 0, 0)]
+    outs = list(range(2, 10)) + [11, 12, 13, 15]
+    p = program(ops, 2, outs)
+    J = p.jacobian([0, 1])
+    x = np.array([0.7, 2.0, 3.0, 1.5])
+    y = np.array([1.3, 0.5, 3.0, 2.5])
+    got = oracle.interpret(J.linear(), np.vstack([x, y]))
+    n = len(outs)
+    d = got[n:].reshape(n, 2, -1)
+    one, zero = np.ones_like(x), np.zeros_like(x)
+    le = x <= y
+    expect = [
+        (y, x), (np.exp(x), zero), (1 / y, -x / y ** 2), (np.where(le, one, zero), np.where(le, zero, one)),
+        (np.where(y <= x, one, zero), np.where(y <= x, zero, one)), (0.5 / np.sqrt(x), zero), (1 / x, zero), (2 * x, zero),
+        (np.where(le, y, exp_or(x)), np.where(le, x, zero)), (-one, zero), (one, -one), (3.5 * one, zero)]
+    for row, (dx, dy) in enumerate(expect):
+        np.testing.assert_allclose(d[row, 0], dx, rtol=1e-15, err_msg=f"row {row} d/dx")
+        np.testing.assert_allclose(d[row, 1], dy, rtol=1e-15, err_msg=f"row {row} d/dy")
+
+
+def exp_or(x):
+    return np.exp(x)
+
+
+def central_differences(lin, X, wrt, h_rel=1e-6):
+    n_out = len(lin["outputs"])
+    J = np.zeros((n_out, len(wrt), X.shape[1]))
+    for j, i in enumerate(wrt):
+        # relative steps of at least h_rel·1e-4, but below half the value, so that a trace species' step does not cross
+        # the max(0, Y_k) clip of the concentrations
+        h = np.minimum(h_rel * np.maximum(np.abs(X[i]), 1e-4), 0.5 * np.abs(X[i]))
+        Xp, Xm = X.copy(), X.copy()
+        Xp[i] += h
+        Xm[i] -= h
+        J[:, j] = (oracle.interpret(lin, Xp) - oracle.interpret(lin, Xm)) / (2 * h)
+    return J
+
+
+def test_reactor_program_matches_the_oracle():
+    """cb_build_reactor: dT/dt of the energy balance and dY_k/dt = W_k ω_k / ρ against the oracle."""
+    M, mech, o = setup("h2o2-10")
+    T, P, Y = synth.config_states("h2o2-10", 64, 95)
+    got = oracle.interpret(M.reactor().linear(), np.vstack([T, P, Y]))
+    for s in range(64):
+        omega, dT, x = o.rhs_full(T[s], P[s], Y[:, s])
+        rho = P[s] / (R * T[s] * np.sum(Y[:, s] / mech["W"]))
+        Ka = mech["Ka"]
+        assert abs(got[0, s] - dT) <= 1e-12 * x["dTdt_scale"]
+        dY = mech["W"][:Ka] * omega / rho
+        scale = mech["W"][:Ka] * x["omega_scale"] / rho
+        assert np.all(np.abs(got[1:1 + Ka, s] - dY) <= 1e-12 * np.maximum(np.abs(dY), scale))
+        assert np.all(got[1 + Ka:, s] == 0)
+
+
+@pytest.mark.parametrize("name,n", [("h2o2-10", 24), ("gri-53", 3)])
+def test_reactor_jacobian_matches_central_differences(name, n):
+    M, mech, o = setup(name)
+    K = M.K
+    T, P, Y = synth.config_states(name, n, 96)
+    T = np.where(np.abs(T - 1000.0) < 5.0, T + 10.0, T)  # away from the NASA-7 range switch, where f is not smooth
+    X = np.vstack([T, P, Y])
+    f = M.reactor()
+    wrt = [0] + list(range(2, 2 + K))  # T and every Y_k
+    J = f.jacobian(wrt)
+    lin = J.linear()
+    got = oracle.interpret(lin, X)
+    m = 1 + K
+    np.testing.assert_array_equal(got[:m], oracle.interpret(f.linear(), X))  # the program's own outputs, unchanged
+    analytic = got[m:].reshape(m, len(wrt), n)
+    fd = central_differences(f.linear(), X, wrt)
+    # floor: each row's largest entry (central differences carry relative errors of ~1e-9 of the row's scale)
+    scale = np.max(np.abs(analytic), axis=1, keepdims=True) + 1e-300
+    err = np.abs(analytic - fd) / np.maximum(np.abs(analytic), scale)
+    print(name, "max error of the analytic Jacobian against central differences", err.max(), "instructions", J.n_instructions)
+    assert err.max() < 1e-5
