@@ -15,6 +15,7 @@
 #include "ignition.hpp"
 #include "integrator.hpp"
 #include "jacobian.hpp"
+#include "noalloc.hpp"
 #include "forms.hpp"
 #include "mechanism.hpp"
 #include "mlp.hpp"
@@ -496,6 +497,19 @@ int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, s
     for (int32_t i = 0; i < n_outputs && n_states; i++) require(outputs[i], "null output array");
     require(n_states < (size_t(1) << 31), "n_states must be below 2^31");
     mod->form->call(inputs, n_states, outputs, pointers_are_device != 0, (hipStream_t)stream, kernel_ms);
+  });
+}
+
+int cb_hip_call_static(cb_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs,
+                       int32_t n_outputs, void* stream) {
+  return guard([&] {
+    require(mod && mod->form, "null module");
+    require(n_inputs == mod->form->n_inputs() && n_outputs == mod->form->n_outputs(), "input/output count does not match the program");
+    require(n_states == 0 || (inputs && outputs), "null array table");
+    for (int32_t i = 0; i < n_inputs && n_states; i++) require(inputs[i], "null input array");
+    for (int32_t i = 0; i < n_outputs && n_states; i++) require(outputs[i], "null output array");
+    require(n_states < (size_t(1) << 31), "n_states must be below 2^31");
+    call_static(*mod->form, inputs, n_states, outputs, (hipStream_t)stream);
   });
 }
 

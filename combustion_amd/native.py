@@ -83,6 +83,7 @@ SIGNATURES = {
     "cb_hip_assemble_export": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "cb_hip_compile": (C.c_int, [_P, C.c_int32, C.c_uint32]),
     "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
+    "cb_hip_call_static": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, _P]),
     "cb_module_free": (None, [_P]),
     "cb_integrator_create": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
     "cb_integrator_advance": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int32, _P, _I64P]),

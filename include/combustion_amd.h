@@ -152,6 +152,16 @@ int cb_hip_compile(const cb_program* p, int32_t form, uint32_t block_size);
  * kernel time measured with HIP events on that stream). */
 int cb_hip_call(cb_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs, int32_t n_outputs,
                 int32_t pointers_are_device, void* stream, float* kernel_ms);
+/* Allocation-free variant of cb_hip_call. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Device pointers
+ * only. The arrays' pointer table, and a surrogate module's fallback counter and state list, live in one static
+ * device array per GPU of 16 KiB + 256 B + 8 MiB, held until the process ends: at most 2048 arrays per module
+ * (CB_ERR_UNSUPPORTED beyond) and at most 2^21 states per call on a surrogate module (CB_ERR_ARGUMENT beyond). The
+ * module's kernels all run on `stream`, after small kernels that write the pointer table (nothing is copied from host
+ * memory, so the call can be captured into a graph). Asynchronous. Every call on one GPU uses that GPU's one array:
+ * calls that overlap on one GPU (different streams, several host threads) race; calls on one stream are ordered. */
+int cb_hip_call_static(cb_module* mod, const double* const* inputs, int32_t n_inputs, size_t n_states, double* const* outputs,
+                       int32_t n_outputs, void* stream);
 void cb_module_free(cb_module* mod);
 
 /* The CPU path: ir/lib.rs:225-263 compile + assemble (the reference's Cranelift JIT of one per-state function,
