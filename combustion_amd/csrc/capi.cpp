@@ -594,6 +594,9 @@ int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w
     copy(sw.o_std, w->output_std);
     require(w->major && w->n_major >= 1 && w->n_major <= NF - 2, "n_major out of range");
     sw.major.assign(w->major, w->major + w->n_major);
+    require(w->rates_form == 0 || w->rates_form == CB_FORM_FAST || w->rates_form == CB_FORM_MIXED,
+            "rates_form is CB_FORM_FAST (or 0) or CB_FORM_MIXED");
+    sw.rates_form = w->rates_form == CB_FORM_MIXED ? CB_FORM_MIXED : CB_FORM_FAST;
     ProgramView v{p->mechanism, &p->function, &p->linear, p->kind, &p->chemistry};
     auto* m = new cb_module;
     try {

@@ -45,6 +45,8 @@ std::string emit_device_function(const ast::Linear& l, const std::vector<std::st
 
 // Mixed-precision rates kernel template (kernels/rates_mixed.hip), embedded at build time.
 extern const char* const rates_mixed_source;
+// FP64 rates kernel for large mechanisms (kernels/rates_tiled.hip).
+extern const char* const rates_tiled_source;
 // Mixture-averaged transport kernel template (kernels/transport_tiled.hip).
 extern const char* const transport_tiled_source;
 // Surrogate network kernels (kernels/mlp.hip).

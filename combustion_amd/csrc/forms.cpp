@@ -57,16 +57,17 @@ StraightOptions straight_options(int form, unsigned block_size) {
 // them, and so does the tiled-transport modifier.
 bool use_tiled(const ProgramView& p, int form_flags) {
   const int form = form_flags & kFormBase;
-  if (form_flags & kFormTiledTransport) {
-    if (form < 1 || form > 3) throw ArgumentError("the tiled-transport modifier applies to forms 1-3");
-    if (!tiled_applies(p)) throw Unsupported("the tiled-transport modifier needs a chemistry program without the reactor right-hand side");
+  if (form_flags & (kFormTiledTransport | kFormTiledRates)) {
+    if (form < 1 || form > 3) throw ArgumentError("the tiled-transport and tiled-rates modifiers apply to forms 1-3");
+    if ((form_flags & kFormTiledRates) && form == 2) throw ArgumentError("the mixed form has its own rates kernel");
+    if (!tiled_applies(p)) throw Unsupported("the tiled modifiers need a chemistry program without the reactor right-hand side");
     return true;
   }
   return form == 2 || ((form == 1 || form == 3) && tiled_applies(p));
 }
 
 void check_form(int form_flags) {
-  if ((form_flags & ~(kFormBase | kFormTiledTransport)) || (form_flags & kFormBase) > 3) throw ArgumentError("unknown form");
+  if ((form_flags & ~(kFormBase | kFormTiledTransport | kFormTiledRates)) || (form_flags & kFormBase) > 3) throw ArgumentError("unknown form");
 }
 
 std::string form_source(const ProgramView& p, int form, unsigned block_size) {

@@ -68,6 +68,7 @@ struct Launch {
 // cb_hip_assemble's form argument: the form in the low bits, modifier flags above (cb_form, cb_form_flags).
 constexpr int kFormBase = 0xF;
 constexpr int kFormTiledTransport = 0x10;
+constexpr int kFormTiledRates = 0x20;
 
 // Common hiprtc options for gfx950 (packed FP32 instructions disabled).
 std::vector<std::string> hip_options(bool contract);
@@ -89,6 +90,7 @@ struct SurrogateWeights {
   int width = 256, depth = 3;
   std::vector<float> params, f_mean, f_rstd, lo, hi, o_mean, o_std;
   std::vector<int32_t> major;  // species whose pair terms the baselines keep
+  int rates_form = 3;          // 3: FP32-core straight-line rates; 2: MFMA mixed-precision rates
 };
 std::unique_ptr<Form> assemble_surrogate(const ProgramView& p, const SurrogateWeights& w);
 unsigned surrogate_fallback_count(Form& f, hipStream_t stream);

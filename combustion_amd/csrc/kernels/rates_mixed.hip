@@ -34,7 +34,7 @@ typedef unsigned short u16;
 #define LN10F 2.302585092994046f
 #define LOG2_FLT_MAX 127.99999991400867  // log2(FLT_MAX): the reference's min(f32::MAX, k), reaction.rs:138
 #define LOG2_FLT_MIN (-126.0f)          // log2(FLT_MIN): the + f32::MIN_POSITIVE guards, reaction.rs:177,189
-#define C_FLOOR 1e-30f                  // concentrations below 1e-30 mol/m³ enter log2 C as 1e-30
+#define LOG2_ABSENT (-16384.f)          // log2 C of an absent species (C <= 0): exact in BF16; 2^x of any product with it is 0
 #define NO_TERM (-1e300)
 
 // Reaction tables (sorted reaction r):
@@ -53,6 +53,16 @@ __device__ __forceinline__ u16 bf16_bits(float x) {
 __device__ __forceinline__ float bf16_value(u16 b) { return __uint_as_float((unsigned)b << 16); }
 __device__ __forceinline__ float log2f_hw(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float exp2f_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+// log2 C in FP32 over the whole FP64 range: the hardware logarithm on FP32-normal values; below FLT_MIN, the binary
+// exponent plus the logarithm of the mantissa; C <= 0 (a species absent from the state) as LOG2_ABSENT, so that the
+// reactions it enters get zero rates, as the reference's products of concentrations do
+__device__ __forceinline__ float log2_concentration(double c) {
+  if (!(c > 0.0)) return LOG2_ABSENT;
+  if (c >= 1.1754943508222875e-38) return log2f_hw((float)c);
+  int e;
+  const double m = __builtin_frexp(c, &e);
+  return (float)e + log2f_hw((float)m);
+}
 // log2(1 + 2^x) in FP32
 __device__ __forceinline__ float softplus2(float x) {
   return x > 0.f ? x + log2f_hw(1.f + exp2f_hw(-x)) : log2f_hw(1.f + exp2f_hw(x));
@@ -203,8 +213,9 @@ rates_mixed(const double* const* __restrict__ in, double* const* __restrict__ ou
   }
   __syncthreads();
   for (int k = grp; k < KB * 32; k += 4) {
-    const float c = k < K ? L.C[k][col] : 0.f;
-    const float x = k < K ? log2f_hw(fmaxf(c, C_FLOOR)) : 0.f;
+    // the FP64 concentration again (L.C holds it in FP32, which flushes values below 1.4e-45)
+    const double c = k < K - 1 ? density * fmax(0.0, (valid ? in[2 + k][s] : 1.0 / K) * rcp_W[k]) : C_tot - partial;
+    const float x = k < K ? log2_concentration(c) : 0.f;
     const u16 h = bf16_bits(x);
     const float r1 = x - bf16_value(h);
     const u16 m = bf16_bits(r1);

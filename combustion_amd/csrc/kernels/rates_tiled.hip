@@ -1,0 +1,177 @@
+// Net production rates in FP64 for mechanisms too large for a straight-line kernel (hand-written; the host prepends the
+// mechanism sizes). A wave owns NS = 8 states and splits the reactions over G = 8 lane groups (lane = 8 group + state):
+//   1. per state, spread over the groups: concentrations C_k (reaction.rs:362-373 on a mass basis), the clamped
+//      exponentials E+_k = min(c+, e^g_k) and E-_k = min(c-, 1/e^g_k) of the Gibbs functions g_k = G°_k/RT of the
+//      state's NASA-7 piece (reaction.rs:236-268), and zeroed production rates, all in LDS ([species][state]);
+//   2. rounds of G reactions, one per group, every reaction of a round of one model (the host sorts and pads them):
+//      k_f from its Arrhenius exponent exp(ln A + β ln T - θ/T) clamped at FLT_MAX (reaction.rs:113-139), the third-
+//      body sum M = C_tot + Σ (e_k - 1) C_k (reaction.rs:141-145), Lindemann or Troe falloff (reaction.rs:149-193);
+//      q = k_f Π C^ν' - k_f (1/K_c) Π C^ν'' with 1/K_c = Π E±^|ν| (P0/RT)^(-Σν) (reaction.rs:271-308);
+//      ω_k += ν_k q by LDS atomic adds (reaction.rs:312-327; the order of the additions differs from the reference's);
+//   3. the rates of the wave's states written out.
+// Tables (row = sorted, padded reaction; see rates_tiled_tables in tiled.cpp):
+//   rate[row] = (ln A, β, θ = Ea/R, -Σν); k0[row] = (ln A0 (Troe: ln(A0/A∞)), β0, θ0, A∞); troe[row] = (α, T3, T1, T2)
+//   (T1 >= 1e30: no α e^(-T/T1) term, α instead; T2 NaN: no e^(-T2/T) term);
+//   info[row] = (model | kinf_const << 4, first efficiency, efficiency count, 0), model 0 elementary, 1 irreversible,
+//   2 three-body, 3 Lindemann, 4 Troe, 5 padding; reactants/products[row] = up to 3 species (with multiplicity, -1
+//   none); net[row][6] = 16 s + (ν + 4) per species changed, -1 none; eff_species/eff_value = the (e_k - 1) lists;
+//   nasa[k] = the 2 x 7 coefficients of g_k (low piece first), split[k] the switch temperature (+inf: one piece).
+// Required defines: K (species), KA (active species), ROUNDS (rounds of G reactions).
+
+#define NS 8
+#define G 8
+#define RGAS (1.380649e-23 * 6.02214076e23)
+#define P0_R (101325.0 / RGAS)
+#define FLT_MAX_D 3.4028234663852886e38
+#define FLT_MIN_D 1.1754943508222875e-38
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// e^x: Cody-Waite reduction by ln 2 in two parts, a degree-10 polynomial on |r| <= ln(2)/2 and an exponent add
+// (combustion_amd/csrc/exp_coefficients.py; relative error below 5e-16), arguments clamped to [-746, 710].
+__device__ __forceinline__ double exp64(double x) {
+  x = __builtin_fmin(__builtin_fmax(x, -746.0), 710.0);
+  const double t = __builtin_fma(x, 0x1.71547652b82fep+0, 0x1.8p+52);
+  const double n = t - 0x1.8p+52;
+  double r = __builtin_fma(-n, 0x1.62e42fee00000p-1, x);
+  r = __builtin_fma(-n, 0x1.a39ef35793c76p-33, r);
+  double p = 0x1.28a2c0a7209fbp-22;
+  p = __builtin_fma(p, r, 0x1.72faf024b693bp-19);
+  p = __builtin_fma(p, r, 0x1.a019a66a75dd4p-16);
+  p = __builtin_fma(p, r, 0x1.a01978c6baf81p-13);
+  p = __builtin_fma(p, r, 0x1.6c16c17f43a58p-10);
+  p = __builtin_fma(p, r, 0x1.1111112dd67c5p-7);
+  p = __builtin_fma(p, r, 0x1.55555555520afp-5);
+  p = __builtin_fma(p, r, 0x1.555555554b757p-3);
+  p = __builtin_fma(p, r, 0x1.0000000000005p-1);
+  p = __builtin_fma(p, r, 0x1.000000000001ep+0);
+  p = __builtin_fma(p, r, 1.0);
+  return __builtin_ldexp(p, (int)(unsigned)__double_as_longlong(t));
+}
+
+// Sum over the G groups of a state (lanes col, col + 8, ..., col + 56).
+__device__ __forceinline__ double group_sum(double v) {
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+struct Lds {
+  double C[K][NS];
+  double Ep[KA][NS];
+  double En[KA][NS];
+  double W[KA][NS];
+};
+
+extern "C" __global__ void __launch_bounds__(64)
+rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n,
+            const f64x4* __restrict__ rate, const f64x4* __restrict__ k0, const f64x4* __restrict__ troe,
+            const i32x4* __restrict__ info, const i32x4* __restrict__ reactants, const i32x4* __restrict__ products,
+            const int* __restrict__ net, const int* __restrict__ eff_species, const double* __restrict__ eff_value,
+            const double* __restrict__ rcp_W, const double* __restrict__ nasa, const double* __restrict__ split,
+            double clamp_pos, double clamp_neg) {
+  __shared__ Lds L;
+  const int lane = threadIdx.x, col = lane & (NS - 1), grp = lane >> 3;
+  const unsigned s = blockIdx.x * NS + col;
+  const bool valid = s < n;
+  // states past n compute on a harmless stand-in and write nothing
+  const double T = valid ? in[0][s] : 1000.0, P = valid ? in[1][s] : 101325.0;
+  const double rcpT = 1.0 / T, lnT = log(T), T2 = T * T;
+  const double C_tot = P * (1.0 / RGAS) * rcpT;
+
+  // 1. concentrations: n_k = Y_k / W_k, C_k = max(0, n_k) C_tot / Σn (bulk species: C_tot - Σ the others)
+  double sum_n = 0.0;
+  for (int k = grp; k < K; k += G) sum_n += (valid ? in[2 + k][s] : 1.0 / K) * rcp_W[k];
+  sum_n = group_sum(sum_n);
+  const double density = C_tot / sum_n;
+  double nonbulk = 0.0;
+  for (int k = grp; k < K - 1; k += G) {
+    const double c = density * fmax(0.0, (valid ? in[2 + k][s] : 1.0 / K) * rcp_W[k]);
+    L.C[k][col] = c;
+    nonbulk += c;
+  }
+  nonbulk = group_sum(nonbulk);
+  if (grp == 0) L.C[K - 1][col] = C_tot - nonbulk;
+  // Gibbs functions of the active species: g = a0 - a6 - a0 ln T - a1/2 T + ... + a5/T, as reaction.rs:78-81 arranges
+  // the coefficients (the host stores them so arranged), one piece by T <= split
+  for (int k = grp; k < KA; k += G) {
+    const double* a = nasa + 14 * k + (T <= split[k] ? 0 : 7);
+    const double g = a[0] + a[1] * lnT + a[2] * T + a[3] * T2 + a[4] * (T2 * T) + a[5] * (T2 * T2) + a[6] * rcpT;
+    const double e = exp64(g);
+    L.Ep[k][col] = fmin(clamp_pos, e);
+    L.En[k][col] = fmin(clamp_neg, 1.0 / e);
+    L.W[k][col] = 0.0;
+  }
+  __syncthreads();
+
+  const double C0 = P0_R * rcpT, rcpC0 = T * (1.0 / P0_R);
+  // 2. rounds of G reactions
+  for (int j = 0; j < ROUNDS; j++) {
+    const int row = G * j + grp;
+    const i32x4 inf = info[row];
+    const int model = inf[0] & 15;  // the round's model, or 5 on the padding rows that end a model's last round
+    if (model == 5) continue;
+    const f64x4 c = rate[row];
+    double kf = fmin(FLT_MAX_D, exp64(c[0] + c[1] * lnT - c[2] * rcpT));
+    if (model >= 2) {
+      double M = C_tot;
+      for (int e = inf[1]; e < inf[1] + inf[2]; e++) M = __builtin_fma(eff_value[e], L.C[eff_species[e]][col], M);
+      if (model == 2) {
+        kf *= M;
+      } else {
+        const f64x4 c0 = k0[row];
+        const double k0v = fmin(FLT_MAX_D, exp64(c0[0] + c0[1] * lnT - c0[2] * rcpT));
+        if (model == 3) {
+          kf = M * k0v * kf / (M * k0v + kf + FLT_MIN_D);  // Lindemann (reaction.rs:153-158)
+        } else {
+          // Troe (reaction.rs:160-193): k0 and k∞ with A∞ factored out (kf holds A∞ k∞' unless k∞ is constant)
+          const bool kinf_const = (inf[0] >> 4) & 1;
+          const double A_inf = c0[3];
+          const double kinf = kinf_const ? 1.0 : fmin(FLT_MAX_D, exp64(c[1] * lnT - c[2] * rcpT));
+          const double Pr = kinf_const ? M * k0v : M * k0v / (kinf + FLT_MIN_D);
+          const f64x4 t = troe[row];
+          double Fcent = (1.0 - t[0]) * exp64(-T / t[1]) + (t[2] < 1e30 ? t[0] * exp64(-T / t[2]) : t[0]);
+          if (t[3] == t[3]) Fcent += exp64(-t[3] * rcpT);
+          const double lnF = log(Fcent);
+          const double C = -0.67 * lnF - 0.4 * 2.302585092994046, N = -1.27 * lnF + 0.75 * 2.302585092994046;
+          const double lp = log(Pr + FLT_MIN_D) + C;
+          const double f1 = lp / (-0.14 * lp + N);
+          const double F = exp64(lnF / (f1 * f1 + 1.0));
+          kf = F * A_inf * kinf * Pr / (Pr + 1.0);
+        }
+      }
+    }
+    const i32x4 re = reactants[row], pr = products[row];
+    double q = kf;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+      if (re[i] >= 0) q *= L.C[re[i]][col];
+    if (model != 1) {
+      double b = kf * (c[3] == 0.0 ? 1.0 : c[3] == 1.0 ? C0 : c[3] == -1.0 ? rcpC0 : rcpC0 * rcpC0);
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        const int e = net[6 * row + i];
+        if (e >= 0) {
+          const int sp = e >> 4, nu = (e & 15) - 4;
+          const double f = nu > 0 ? L.Ep[sp][col] : L.En[sp][col];
+          for (int m = 0; m < (nu > 0 ? nu : -nu); m++) b *= f;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+        if (pr[i] >= 0) b *= L.C[pr[i]][col];
+      q -= b;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      const int e = net[6 * row + i];
+      if (e >= 0) atomicAdd(&L.W[e >> 4][col], (double)((e & 15) - 4) * q);
+    }
+  }
+  __syncthreads();
+  // 3. outputs
+  if (valid)
+    for (int k = grp; k < KA; k += G) out[k][s] = L.W[k][col];
+}

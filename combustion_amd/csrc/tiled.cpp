@@ -205,6 +205,132 @@ std::string rates_mixed_text(const MixedTables& t) {
   return s.str();
 }
 
+// Tables of kernels/rates_tiled.hip (layouts described there): rows sorted by model, each model's rows padded to whole
+// rounds of 8 so that a round is of one model.
+struct TiledRatesTables {
+  int K, KA, ROUNDS;
+  std::vector<double> rate, k0, troe, eff_value, rcp_W, nasa, split;
+  std::vector<int32_t> info, reactants, products, net, eff_species;
+};
+
+TiledRatesTables rates_tiled_tables(const Mechanism& m) {
+  TiledRatesTables t;
+  const Species& sp = m.species;
+  t.K = (int)sp.len();
+  t.KA = (int)m.active;
+  const int R = (int)m.reactions.size(), G = 8;
+  auto model_code = [](const Reaction& x) {
+    if (!x.plog.empty()) throw Unsupported("the tiled FP64 rates kernel has no pressure-dependent Arrhenius tables (the straight-line forms do)");
+    switch (x.model) {
+      case ReactionModel::Elementary: return 0;
+      case ReactionModel::Irreversible: return 1;
+      case ReactionModel::ThreeBody: return 2;
+      case ReactionModel::PressureModification: return 3;
+      case ReactionModel::Troe: return 4;
+      default: throw Unsupported("the tiled FP64 rates kernel has no SRI falloff model (the straight-line forms do)");
+    }
+  };
+  std::vector<int> order(R);
+  for (int r = 0; r < R; r++) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return model_code(m.reactions[a]) < model_code(m.reactions[b]); });
+  std::vector<int> rows;
+  for (size_t i = 0; i < order.size();) {
+    const int code = model_code(m.reactions[order[i]]);
+    while (i < order.size() && model_code(m.reactions[order[i]]) == code) rows.push_back(order[i++]);
+    while (rows.size() % G) rows.push_back(-1);
+  }
+  if (rows.empty()) rows.assign(G, -1);
+  t.ROUNDS = (int)rows.size() / G;
+  const size_t RP = rows.size();
+  t.rate.assign(4 * RP, 0.);
+  t.k0.assign(4 * RP, 0.);
+  t.troe.assign(4 * RP, 0.);
+  t.info.assign(4 * RP, 0);
+  t.reactants.assign(4 * RP, -1);
+  t.products.assign(4 * RP, -1);
+  t.net.assign(6 * RP, -1);
+  for (size_t r = 0; r < RP; r++) {
+    if (rows[r] < 0) {
+      t.info[4 * r] = 5;
+      continue;
+    }
+    const Reaction& x = m.reactions[rows[r]];
+    const int code = model_code(x);
+    const RateConstant& k = x.rate_constant;
+    if (code != 1 && (-x.sum_net < -2 || -x.sum_net > 1))
+      throw Unsupported("unsupported change in the number of moles: " + x.equation);
+    t.rate[4 * r] = std::log(k.preexponential_factor);
+    t.rate[4 * r + 1] = k.temperature_exponent;
+    t.rate[4 * r + 2] = k.activation_temperature;
+    t.rate[4 * r + 3] = -x.sum_net;
+    int kinf_const = 0;
+    if (code >= 3) {
+      // Troe factors A∞ out of k0 (reaction.rs:163-170); Lindemann takes both limits whole
+      t.k0[4 * r] = std::log(code == 4 ? x.k0.preexponential_factor / k.preexponential_factor : x.k0.preexponential_factor);
+      t.k0[4 * r + 1] = x.k0.temperature_exponent;
+      t.k0[4 * r + 2] = x.k0.activation_temperature;
+      t.k0[4 * r + 3] = k.preexponential_factor;
+    }
+    if (code == 4) {
+      kinf_const = k.temperature_exponent == 0. && k.activation_temperature == 0.;
+      t.troe[4 * r] = x.troe.A;
+      t.troe[4 * r + 1] = x.troe.T3;
+      t.troe[4 * r + 2] = x.troe.T1;
+      t.troe[4 * r + 3] = std::isfinite(x.troe.T2) ? x.troe.T2 : NAN;
+    }
+    t.info[4 * r] = code | (kinf_const << 4);
+    if (code >= 2) {
+      t.info[4 * r + 1] = (int32_t)t.eff_species.size();
+      for (int s = 0; s < t.K; s++)
+        if (x.efficiencies[s] != 1.) {
+          t.eff_species.push_back(s);
+          t.eff_value.push_back(x.efficiencies[s] - 1.);
+        }
+      t.info[4 * r + 2] = (int32_t)t.eff_species.size() - t.info[4 * r + 1];
+    }
+    auto side = [&](const std::vector<uint8_t>& nu, std::vector<int32_t>& out) {
+      int n = 0;
+      for (int s = 0; s < t.K; s++)
+        for (int c = 0; c < nu[s]; c++) {
+          if (n == 3) throw Unsupported("more than 3 reactants or products in reaction " + x.equation);
+          out[4 * r + n++] = s;
+        }
+    };
+    side(x.reactants, t.reactants);
+    side(x.products, t.products);
+    int n = 0;
+    for (size_t s = 0; s < x.net.size(); s++) {
+      const int nu = x.net[s];
+      if (!nu) continue;
+      if (n == 6 || std::abs(nu) > 4) throw Unsupported("too many or too large stoichiometric changes in reaction " + x.equation);
+      t.net[6 * r + n++] = (int32_t)s * 16 + (nu + 4);
+    }
+  }
+  if (t.eff_species.empty()) {  // keep the device arrays non-empty
+    t.eff_species.push_back(0);
+    t.eff_value.push_back(0.);
+  }
+  for (int k = 0; k < t.K; k++) t.rcp_W.push_back(1. / sp.molar_mass[k]);
+  for (int k = 0; k < t.KA; k++) {
+    const NASA7& nasa = sp.thermodynamics[k];
+    for (int p = 0; p < 2; p++) {
+      const auto& a = nasa.pieces[p];
+      const double g[7] = {a[0] - a[6], -a[0], -a[1] / 2., (1. / 3. - 1. / 2.) * a[2], (1. / 4. - 1. / 3.) * a[3], (1. / 5. - 1. / 4.) * a[4], a[5]};
+      t.nasa.insert(t.nasa.end(), g, g + 7);
+    }
+    t.split.push_back(std::isnan(nasa.temperature_split) ? INFINITY : nasa.temperature_split);
+  }
+  const size_t lds = ((size_t)t.K + 3 * (size_t)t.KA) * 8 * 8;
+  if (lds > 64 * 1024) throw Unsupported("mechanism too large for the tiled FP64 rates kernel's LDS");
+  return t;
+}
+
+std::string rates_tiled_text(const TiledRatesTables& t) {
+  std::ostringstream s;
+  s << "#define K " << t.K << "\n#define KA " << t.KA << "\n#define ROUNDS " << t.ROUNDS << "\n" << rates_tiled_source;
+  return s.str();
+}
+
 // Tables of kernels/transport_tiled.hip: unscaled fits (transport.rs:125-136), W^1/4, the Wilke matrix a² and the
 // diffusion-fit tiles in the lane layout of v_mfma_{f64,f32}_16x16x4 (A[row L%16][col L/16]).
 struct TransportTables {
@@ -358,6 +484,48 @@ std::string rates_straight_text(const ProgramView& p, bool fast) {
   return emit_straight(l, fast ? "rates_fast" : "rates_fp64", so);
 }
 
+// rates_tiled with its tables in device memory; writes ω to out[0..Ka).
+class RatesTiled {
+ public:
+  explicit RatesTiled(const Mechanism& m) : t_(rates_tiled_tables(m)) {
+    kernel_.load(device::compile(rates_tiled_text(t_), "rates_tiled", hip_options(true)), "rates_tiled");
+    d_rate_ = upload(t_.rate);
+    d_k0_ = upload(t_.k0);
+    d_troe_ = upload(t_.troe);
+    d_info_ = upload(t_.info);
+    d_reactants_ = upload(t_.reactants);
+    d_products_ = upload(t_.products);
+    d_net_ = upload(t_.net);
+    d_eff_species_ = upload(t_.eff_species);
+    d_eff_value_ = upload(t_.eff_value);
+    d_rcp_W_ = upload(t_.rcp_W);
+    d_nasa_ = upload(t_.nasa);
+    d_split_ = upload(t_.split);
+  }
+  ~RatesTiled() {
+    for (void* ptr : {(void*)d_rate_, (void*)d_k0_, (void*)d_troe_, (void*)d_info_, (void*)d_reactants_, (void*)d_products_, (void*)d_net_,
+                      (void*)d_eff_species_, (void*)d_eff_value_, (void*)d_rcp_W_, (void*)d_nasa_, (void*)d_split_})
+      if (ptr) (void)hipFree(ptr);
+  }
+  void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
+    unsigned n32 = (unsigned)n;
+    double pos = 6981463048192., neg = (double)cbrtf(3.40282347e38f);  // reaction.rs:244-256
+    void* args[] = {&in, &out, &n32, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_reactants_, &d_products_, &d_net_, &d_eff_species_,
+                    &d_eff_value_, &d_rcp_W_, &d_nasa_, &d_split_, &pos, &neg};
+    const unsigned states_per_block = 8;
+    device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + states_per_block - 1) / states_per_block), 1, 1, 64, 1, 1, 0,
+                                        stream, args, nullptr),
+                  "launch rates_tiled");
+  }
+
+ private:
+  TiledRatesTables t_;
+  device::Kernel kernel_;
+  double *d_rate_ = nullptr, *d_k0_ = nullptr, *d_troe_ = nullptr, *d_eff_value_ = nullptr, *d_rcp_W_ = nullptr, *d_nasa_ = nullptr,
+         *d_split_ = nullptr;
+  int32_t *d_info_ = nullptr, *d_reactants_ = nullptr, *d_products_ = nullptr, *d_net_ = nullptr, *d_eff_species_ = nullptr;
+};
+
 // rates_mixed with its tables in device memory; writes ω to out[0..Ka).
 class RatesMixed {
  public:
@@ -422,6 +590,11 @@ class RatesStraight {
 // Mechanisms up to 64 species take the transport section as straight-line code (one lane per state): at K=53 it
 // issues fewer instructions per state than the tiled kernel, whose MFMA contractions do not pay for the
 // reciprocals and reductions around them. Larger mechanisms (a K² program) take transport_tiled.
+// Mechanisms above kMaxStraightRatesSpecies (or any, with kFormTiledRates) take rates_tiled in forms 1 and 3.
+bool tiled_rates(const ProgramView& p, int form) {
+  return p.mechanism->species.len() > kMaxStraightRatesSpecies || (form & kFormTiledRates);
+}
+
 // form | kFormTiledTransport sends every mechanism through transport_tiled (the MFMA contractions) instead.
 bool straight_transport(const ProgramView& p, int form) { return p.mechanism->species.len() <= 64 && !(form & kFormTiledTransport); }
 
@@ -470,6 +643,8 @@ class Tiled : public SoAForm {
       n_rates_ = (int)p.mechanism->active;
       if (form == 2)
         mixed_ = std::make_unique<RatesMixed>(*p.mechanism);
+      else if (tiled_rates(p, form_flags))
+        tiled_rates_ = std::make_unique<RatesTiled>(*p.mechanism);
       else
         fp64_ = std::make_unique<RatesStraight>(p, form == 3);
     }
@@ -493,7 +668,7 @@ class Tiled : public SoAForm {
   // the two kernels' blocks share the CUs at the ends of each launch (2.27 against 2.32 ms per 2^20 states at K=53).
   void launch(const double* const* in, double* const* out, size_t n, const Launch& how) override {
     hipStream_t stream = how.stream;
-    const bool both = (mixed_ || fp64_) && (straight_ || transport_) && !how.single_stream;
+    const bool both = (mixed_ || fp64_ || tiled_rates_) && (straight_ || transport_) && !how.single_stream;
     hipStream_t ts = stream;
     if (both) {
       if (!side_) {
@@ -508,6 +683,7 @@ class Tiled : public SoAForm {
     // The longer rates kernel goes first (the other order measured 2.29 against 2.25 ms).
     if (mixed_) mixed_->launch(in, out, n, stream);
     if (fp64_) fp64_->launch(in, out, n, stream);
+    if (tiled_rates_) tiled_rates_->launch(in, out, n, stream);
     if (straight_) straight_->launch(in, out + n_rates_, n, ts);
     if (transport_) transport_->launch(in, out + n_rates_, n, ts);
     if (both) {
@@ -519,6 +695,7 @@ class Tiled : public SoAForm {
  private:
   std::unique_ptr<RatesMixed> mixed_;
   std::unique_ptr<RatesStraight> fp64_;
+  std::unique_ptr<RatesTiled> tiled_rates_;
   std::unique_ptr<TransportStraight> straight_;
   std::unique_ptr<TransportTiled> transport_;
   int n_rates_ = 0;
@@ -540,8 +717,8 @@ std::string surrogate_text(int K, const MlpLayout& lay, int DK, int NM) {
   return s.str();
 }
 
-// FP32-core straight-line rates, then the transport surrogate with the FP32 transport kernel on the states outside
-// its box.
+// FP32-core straight-line rates (or the MFMA mixed rates), then the transport surrogate with the plain FP64 transport
+// kernel on the states outside its box.
 class Surrogate : public SoAForm {
  public:
   Surrogate(const ProgramView& p, const SurrogateWeights& w) : SoAForm(p.function->n_input, (int)p.function->output.size()) {
@@ -557,9 +734,14 @@ class Surrogate : public SoAForm {
       if ((int)v->size() != NF) throw ArgumentError("surrogate normalisation arrays need K+2 entries");
     if (o.rates) {
       n_rates_ = (int)m.active;
-      rates_ = std::make_unique<RatesStraight>(p, true);
+      if (w.rates_form == 2)
+        mixed_ = std::make_unique<RatesMixed>(m);
+      else
+        rates_ = std::make_unique<RatesStraight>(p, true);
     }
-    transport_ = std::make_unique<TransportTiled>(p, false);
+    // The fallback for states outside the network's box: the plain FP64 transport kernel (the tiled one, which
+    // evaluates a list of states in place).
+    transport_ = std::make_unique<TransportTiled>(p, true);
     const Polynomials& poly = polynomials(p);
     const int D = poly.D, DK = D;
     if (D > 8) throw Unsupported("transport fits longer than 8 coefficients");
@@ -666,7 +848,7 @@ class Surrogate : public SoAForm {
     // The rates kernel on the caller's stream; the surrogate and its fallback on a side stream forked from and joined
     // back into it (the two read the same inputs and write disjoint outputs), as in the tiled forms.
     hipStream_t ts = stream;
-    const bool fork = rates_ && !how.single_stream;
+    const bool fork = (rates_ || mixed_) && !how.single_stream;
     if (fork) {
       if (!side_) {
         device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
@@ -678,6 +860,7 @@ class Surrogate : public SoAForm {
       ts = side_;
     }
     if (rates_) rates_->launch(in, out, n, stream);
+    if (mixed_) mixed_->launch(in, out, n, stream);
     double* const* tout = out + n_rates_;
     void* clear_args[] = {&count};
     device::check(hipModuleLaunchKernel(clear_, 1, 1, 1, 64, 1, 1, 0, ts, clear_args, nullptr), "launch surrogate_clear");
@@ -698,6 +881,7 @@ class Surrogate : public SoAForm {
 
  private:
   std::unique_ptr<RatesStraight> rates_;
+  std::unique_ptr<RatesMixed> mixed_;
   std::unique_ptr<TransportTiled> transport_;
   int n_rates_ = 0;
   device::Kernel kernel_;
@@ -723,7 +907,10 @@ std::string tiled_source(const ProgramView& p, int form_flags, unsigned) {
   const int form = form_flags & kFormBase;
   const bool mixed = form != 1;
   std::string s;
-  if (p.chemistry->options.rates) s += form == 2 ? rates_mixed_text(mixed_tables(*p.mechanism)) : rates_straight_text(p, form == 3);
+  if (p.chemistry->options.rates)
+    s += form == 2                      ? rates_mixed_text(mixed_tables(*p.mechanism))
+         : tiled_rates(p, form_flags) ? rates_tiled_text(rates_tiled_tables(*p.mechanism))
+                                      : rates_straight_text(p, form == 3);
   if (p.chemistry->options.transport)
     s += "\n// ---- transport ----\n" + (straight_transport(p, form_flags) ? transport_straight_text(p, !mixed)
                                                          : transport_text(transport_tables(*p.mechanism, polynomials(p), p.chemistry->scales.T0), !mixed));
@@ -737,6 +924,8 @@ void compile_tiled(const ProgramView& p, int form_flags, unsigned) {
   if (p.chemistry->options.rates) {
     if (form == 2)
       device::compile(rates_mixed_text(mixed_tables(*p.mechanism)), "rates_mixed", hip_options(true));
+    else if (tiled_rates(p, form_flags))
+      device::compile(rates_tiled_text(rates_tiled_tables(*p.mechanism)), "rates_tiled", hip_options(true));
     else
       device::compile(rates_straight_text(p, form == 3), form == 3 ? "rates_fast" : "rates_fp64", hip_options(true));
   }

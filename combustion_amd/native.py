@@ -22,17 +22,24 @@ LIB_PATH = os.path.join(HERE, "libcombustion_amd.so")
 RATES, TRANSPORT, RHS, LEAN = 1, 2, 4, 8
 FORM_STRAIGHT, FORM_FP64, FORM_MIXED, FORM_FAST = 0, 1, 2, 3
 FORMS = {"straight": FORM_STRAIGHT, "fp64": FORM_FP64, "mixed": FORM_MIXED, "fast": FORM_FAST}
-FORM_TILED_TRANSPORT = 0x10  # cb_form_flags: "fp64+tiled" etc. send the transport section through the MFMA kernel
+# cb_form_flags: "fp64+tiled" sends the transport section through the MFMA kernel, "fp64+tiled-rates" the rates
+# section through the table-driven kernel of large mechanisms, "fp64+tiled+tiled-rates" both.
+FORM_TILED_TRANSPORT, FORM_TILED_RATES = 0x10, 0x20
+MODIFIERS = {"tiled": FORM_TILED_TRANSPORT, "tiled-rates": FORM_TILED_RATES}
 
 
 def form_code(form):
-    """cb_hip_assemble's form argument from a name: "straight", "fp64", "mixed", "fast", optionally with "+tiled"."""
+    """cb_hip_assemble's form argument from a name: "straight", "fp64", "mixed", "fast", with "+tiled" and/or
+    "+tiled-rates" modifiers."""
     if isinstance(form, int):
         return form
-    base, _, modifier = form.partition("+")
-    if base not in FORMS or modifier not in ("", "tiled"):
+    base, *mods = form.split("+")
+    if base not in FORMS or any(m not in MODIFIERS for m in mods):
         raise ValueError(f"unknown form {form!r}")
-    return FORMS[base] | (FORM_TILED_TRANSPORT if modifier else 0)
+    code = FORMS[base]
+    for m in mods:
+        code |= MODIFIERS[m]
+    return code
 
 STATUS = {0: "ok", -1: "argument", -2: "parse", -3: "unsupported", -4: "build", -5: "compile", -6: "device", -7: "size"}
 
@@ -474,7 +481,7 @@ class SurrogateWeights(C.Structure):
     """cb_surrogate_weights."""
     _fields_ = [("width", C.c_int32), ("depth", C.c_int32)] + [
         (name, C.POINTER(C.c_float)) for name in ("params", "feature_mean", "feature_rstd", "lower", "upper", "output_mean", "output_std")] + [
-        ("n_major", C.c_int32), ("major", C.POINTER(C.c_int32))]
+        ("n_major", C.c_int32), ("major", C.POINTER(C.c_int32)), ("rates_form", C.c_int32)]
 
 
 def _tensor(x, dtype):

@@ -223,8 +223,11 @@ def _hard_pool(net, p, p16, Xd, Td, scale, NF, batch, ws, y, gen, sample=1 << 19
     return idx[torch.topk(e, k).indices]
 
 
-def assemble(program, trained):
-    """The surrogate form of a chemistry program (cb_hip_assemble_surrogate) as a native.Module."""
+def assemble(program, trained, rates="fast"):
+    """The surrogate form of a chemistry program (cb_hip_assemble_surrogate) as a native.Module; rates is "fast"
+    (the FP32-core straight-line rates kernel) or "mixed" (the MFMA mixed-precision one)."""
+    if rates not in ("fast", "mixed"):
+        raise ValueError("rates is 'fast' or 'mixed'")
     K = program.mechanism.K
     arrays = [np.ascontiguousarray(getattr(trained, k), np.float32) for k in Trained.FIELDS]
     for a in arrays[1:]:
@@ -232,7 +235,7 @@ def assemble(program, trained):
             raise ValueError("normalisation arrays need K+2 entries")
     fp = C.POINTER(C.c_float)
     w = native.SurrogateWeights(trained.width, trained.depth, *(a.ctypes.data_as(fp) for a in arrays), len(trained.major),
-                                trained.major.ctypes.data_as(C.POINTER(C.c_int32)))
+                                trained.major.ctypes.data_as(C.POINTER(C.c_int32)), native.FORMS[rates])
     h = native._P()
     native._check(native.lib().cb_hip_assemble_surrogate(program._h, C.byref(w), C.byref(h)))
     mod = native.Module(h, program)

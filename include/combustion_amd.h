@@ -41,15 +41,19 @@ enum cb_status {
 /* Program forms for cb_hip_assemble. */
 enum cb_form {
   CB_FORM_STRAIGHT = 0, /* the program emitted as straight-line HIP, one lane per state (cuda.rs:101-134) */
-  CB_FORM_FP64 = 1,     /* chemistry programs: FP64 rates and transport sections as separate kernels (straight-line rates;
-                           straight-line transport up to 64 species, the tiled MFMA transport kernel above) */
+  CB_FORM_FP64 = 1,     /* chemistry programs: FP64 rates and transport sections as separate kernels (straight-line rates
+                           and transport up to 64 species; the table-driven rates kernel and the tiled MFMA transport
+                           kernel above) */
   CB_FORM_MIXED = 2,    /* tiled mixed precision: BF16-split MFMA contractions, FP32 exp/log, FP64 ΔG/RT */
   CB_FORM_FAST = 3      /* FP64 straight-line rates with FP32 cores for exp and ln, FP32 transport */
 };
 /* Modifier ORed into the form of a chemistry program (forms 1-3): the transport section takes the tiled MFMA
  * kernel (Wilke sums and binary-diffusion polynomials as v_mfma_{f64,f32}_16x16x4 contractions) at any species
  * count, instead of the straight-line transport kernel that forms 1 and 3 use up to 64 species. */
-enum cb_form_flags { CB_FORM_TILED_TRANSPORT = 0x10 };
+enum cb_form_flags { CB_FORM_TILED_TRANSPORT = 0x10,
+/* Modifier for forms 1 and 3: the rates section takes the table-driven FP64 kernel (8 states x 8 reaction groups per
+ * wave, per-state arrays in LDS) that those forms use above 64 species, instead of the straight-line rates kernel. */
+                     CB_FORM_TILED_RATES = 0x20 };
 
 /* Sections of the per-state program built by cb_build_chemistry. CB_LEAN selects the shapes that the
  * one-lane-per-state FP64 kernels compile (the same values to a few ulps per operation): rate-of-progress products
@@ -197,8 +201,9 @@ void cb_integrator_free(cb_integrator* it);
  * plus its BF16 copy, both device memory: per layer W [out][in] then b [out], with in/out padded to multiples
  * of 64 (cb_mlp_sizes). Activations are [n][in_pad] / [n][out_pad] FP32 row-major device arrays. */
 /* Surrogate transport form of a chemistry program (inputs [T, P, Y], CB_TRANSPORT with or without CB_RATES):
- * ω by the FP32-core rates kernel (CB_FORM_FAST), and λ, μ, D_k by the trained network (kernels/surrogate.hip),
- * with the FP32 transport kernel evaluating the states whose features fall outside the training box [lower, upper].
+ * ω by the FP32-core rates kernel (CB_FORM_FAST) or the MFMA mixed-precision one (CB_FORM_MIXED), and λ, μ, D_k by the
+ * trained network (kernels/surrogate.hip), with the plain FP64 transport kernel (the MFMA-tiled one) evaluating the
+ * states whose features fall outside the training box [lower, upper].
  * No counterpart in the reference. The network has n_in = n_out = K+2 (cb_mlp layout); features are
  * (T, ln P, X_0..X_{K-1}) (mole fractions) standardised by (mean, 1/std); outputs are [λ, μ, D_0..D_{K-1}] as
  * ln(property / baseline) standardised by (mean, std), the baselines costing O(K·n_major) per state: the
@@ -215,6 +220,8 @@ typedef struct cb_surrogate_weights {
   const float* output_std;      /* [K+2] */
   int32_t n_major;              /* 1..K, at least one major species other than the bulk */
   const int32_t* major;         /* [n_major] distinct species indices */
+  int32_t rates_form;           /* the rates kernel beside the network: CB_FORM_FAST (or 0) for the FP32-core
+                                   straight-line kernel, CB_FORM_MIXED for the MFMA mixed-precision kernel */
 } cb_surrogate_weights;
 int cb_hip_assemble_surrogate(const cb_program* p, const cb_surrogate_weights* w, cb_module** out);
 /* States of the last cb_hip_call on a surrogate module that went to the plain transport kernel (waits for the

@@ -43,3 +43,33 @@ def test_exact_form_keeps_library_arithmetic():
     M, mech, o = setup("h2o2-10")
     text = M.chemistry(3).hip_source("straight")
     assert "exp64(" not in text and "rcp64(" not in text and "lds_v" not in text
+
+
+def test_large_mechanisms_take_the_table_driven_rates_kernel():
+    """Above 64 species the FP64 and FP32-core forms take rates_tiled (tables, not straight-line code) and the MFMA
+    transport kernel; the modifier selects rates_tiled at any size."""
+    M, mech, o = setup("large-160")
+    for form in ("fp64", "fast"):
+        text = M.chemistry(3).hip_source(form)
+        assert "rates_tiled(" in text and "transport_tiled(" in text and "rates_fp64" not in text
+        assert "#define K 160\n" in text and "#define KA 159\n" in text
+    M, mech, o = setup("gri-53")
+    text = M.chemistry(3).hip_source("fp64+tiled-rates")
+    assert "rates_tiled(" in text and "rates_fp64" not in text
+    assert "rates_fp64" in M.chemistry(3).hip_source("fp64")
+
+
+def test_table_driven_rates_kernel_refuses_what_it_has_no_tables_for():
+    import pytest
+    from combustion_amd import native
+    from combustion_amd.synth import variant_text
+    from helpers import setup_text
+    for variant in ("sri", "plog"):
+        M, mech, o = setup_text(variant_text(variant, "h2o2-10"))
+        with pytest.raises(native.Error, match="tiled FP64 rates kernel has no"):
+            M.chemistry(3).hip_source("fp64+tiled-rates")
+    M, mech, o = setup("h2o2-10")
+    with pytest.raises(native.Error, match="mixed form has its own rates kernel"):
+        M.chemistry(3).hip_source("mixed+tiled-rates")
+    with pytest.raises(ValueError):
+        M.chemistry(3).hip_source("fp64+rates")

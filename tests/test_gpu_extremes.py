@@ -29,9 +29,9 @@ def extreme_states(K, seed):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["h2o2-10", "gri-53"])
-# The mixed (MFMA) form is not among them: its log-domain contraction takes concentrations below 1e-30 mol/m³ as
-# 1e-30, so on the trace states here, where every reaction term involves a trace species, it reports rates of
-# ~1e-12 against true ones of ~1e-286; its domain is stated with its error bound.
+# The mixed (MFMA) form is not among them: its log-domain contraction carries log2 C in FP32, whose rounding at the
+# 1e-300 traces here (|log2 C| ~ 1000, steps of 2^-14) is ~2e-5 per factor, above its 1e-5 bound; exact zeros it
+# handles (test_gpu_parity.py::test_mixed_forms_on_states_with_absent_species).
 @pytest.mark.parametrize("form", ["fp64", "fast"])
 def test_extreme_states_match_oracle(name, form, gpu):
     M, mech, o = setup(name)

@@ -36,8 +36,10 @@ def test_fp64_matches_oracle(name, n, flags, form, gpu):
     if name == "large-160" and form == "straight":
         pytest.skip("the whole K=160 program as one straight-line kernel is not compiled (the fp64 form covers K=160)")
     if name == "large-160" and form == "fp64":
-        pytest.skip("a run of this case faulted the GPU (memory aperture violation) with an earlier transport kernel; the "
-                    "cause is not established, so the case stays off the GPU (see the design notes)")
+        pytest.skip("a run of this case, when it took the straight-line rates kernel (now refused above 64 species) and "
+                    "an earlier transport kernel, faulted the GPU (memory aperture violation) for a cause not established; "
+                    "it stays off the GPU, and test_fp64_at_160_species covers K=160 in FP64 with the kernels that serve it "
+                    "now")
     M, mech, o = setup(name)
     T, P, Y = synth.config_states(name, n, 7)
     rates = bool(flags & 1)
@@ -46,6 +48,21 @@ def test_fp64_matches_oracle(name, n, flags, form, gpu):
     assert np.all(np.isfinite(out))
     err = relative_error(out, ref, floor)
     print(name, form, "max error per section:", {k: float(v.max()) for k, v in sections(err, mech["Ka"], rates).items()}, flush=True)
+    assert err.max() <= TOL
+
+
+@pytest.mark.gpu
+def test_fp64_at_160_species(gpu):
+    """K=160 in the FP64 form: the table-driven rates kernel (rates_tiled) and the MFMA transport kernel
+    (transport_tiled on v_mfma_f64_16x16x4f64), to 1e-12 of the oracle."""
+    M, mech, o = setup("large-160")
+    T, P, Y = synth.config_states("large-160", 2048, 7)
+    ref, floor = o.batch(T, P, Y, threads=8)
+    out = run(M.chemistry(3), "fp64", T, P, Y, gpu)
+    assert np.all(np.isfinite(out))
+    err = relative_error(out, ref, floor)
+    print("large-160 fp64 max error per section:", {k: float(v.max()) for k, v in sections(err, mech["Ka"], True).items()},
+          flush=True)
     assert err.max() <= TOL
 
 
@@ -96,6 +113,44 @@ def test_mixed_matches_oracle_within_stated_error(name, n, flags, form, gpu):
           "fraction of ω under the state floor", below, flush=True)
     if rates:
         assert err[:Ka].max() <= MIXED_RATES_TOL
+    assert err[Ka:].max() <= MIXED_TRANSPORT_TOL
+
+
+def states_with_absent_species(name, n, seed):
+    """Configuration states with species set to exactly zero, as in a fresh mixture or states stored in FP32: every
+    second state loses a random half of its species, every third keeps only two of them besides the bulk species."""
+    T, P, Y = synth.config_states(name, n, seed)
+    K = Y.shape[0]
+    rng = np.random.default_rng(seed + 1)
+    for s in range(0, n, 2):
+        Y[rng.choice(K - 1, (K - 1) // 2, replace=False), s] = 0.0
+    for s in range(0, n, 3):
+        keep = rng.choice(K - 1, 2, replace=False)
+        kept = Y[keep, s].copy()
+        Y[:K - 1, s] = 0.0
+        Y[keep, s] = np.maximum(kept, 1e-3)
+    Y /= Y.sum(axis=0, keepdims=True)
+    return T, P, Y
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["h2o2-10", "gri-53"])
+@pytest.mark.parametrize("form", ["mixed", "fast"])
+def test_mixed_forms_on_states_with_absent_species(name, form, gpu):
+    """Exact zeros in the composition: a reaction with an absent reactant (product) has no forward (reverse) rate,
+    as the reference's product of concentrations gives; the mixed form's log-domain contraction must not replace the
+    zero by a small concentration."""
+    M, mech, o = setup(name)
+    T, P, Y = states_with_absent_species(name, 4096, 11)
+    ref, floor = o.batch(T, P, Y, threads=8)
+    out = run(M.chemistry(), form, T, P, Y, gpu)
+    assert np.all(np.isfinite(out))
+    Ka = mech["Ka"]
+    with np.errstate(invalid="ignore"):
+        err = np.where(out == ref, 0.0, relative_error(out, ref, state_floor(floor, Ka)))
+    print(name, form, "max/median:", {k: (float(v.max()), float(np.median(v))) for k, v in sections(err, Ka, True).items()},
+          flush=True)
+    assert err[:Ka].max() <= MIXED_RATES_TOL
     assert err[Ka:].max() <= MIXED_TRANSPORT_TOL
 
 

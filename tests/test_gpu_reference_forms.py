@@ -155,6 +155,27 @@ def test_tiled_transport_at_small_species_counts(name, form, gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("spec", ["h2o2-10", "gri-53", "models:h2o2-10", "models:gri-53", "handwritten"])
+def test_tiled_rates_at_small_species_counts(spec, gpu):
+    """The table-driven FP64 rates kernel that serves mechanisms above 64 species (8 states × 8 reaction groups per
+    wave, concentrations and clamped exp(±G/RT) in LDS, ω by LDS atomic adds) on mechanisms that the straight-line
+    kernel serves by default, every rate model it takes among them (elementary, irreversible, three-body, Lindemann,
+    Troe with and without a constant k∞): to 1e-12 of the oracle."""
+    if spec == "handwritten":
+        M, mech, o = setup_text(open(os.path.join(ROOT, "tests", "golden", "handwritten.yaml")).read())
+        T, P, Y = synth.states(M.K, 2048, 13, (300.0, 3000.0), (0.1, 50.0), 0.5)
+    else:
+        variant, _, name = spec.rpartition(":")
+        M, mech, o = setup_text(models_variant(name)) if variant else setup(name)
+        T, P, Y = synth.config_states(name, 4096, 76)
+    ref, floor = o.batch(T, P, Y, threads=8)
+    src = M.chemistry().hip_source("fp64+tiled-rates")
+    assert "rates_tiled" in src and "rates_fp64" not in src
+    got = run(M.chemistry(), "fp64+tiled-rates", [T, P, *Y], gpu)
+    check(got, ref, floor, M.active, "fp64", spec)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["h2o2-10", "gri-53"])
 def test_exported_device_functions_on_the_gpu(name, gpu):
     """The nekRK export (cuda.rs:296-337: rates.c, conductivity.c, viscosity.c, diffusivity.c) compiled by hiprtc

@@ -35,8 +35,8 @@ def small(gpu):
 
 @pytest.mark.gpu
 def test_trajectories_train_and_fallback(gpu, small):
-    """The loss falls, and states outside the training box come back from the plain FP32 transport kernel (within
-    its FP32 accuracy of the oracle)."""
+    """The loss falls, and states outside the training box come back from the plain FP64 transport kernel (the
+    FP64 forms' 1e-12 of the oracle)."""
     M, mech, o = setup("gri-53")
     T, P, Y, trained = small
     first, last = trained.history[0][1], trained.history[-1][1]
@@ -47,7 +47,7 @@ def test_trajectories_train_and_fallback(gpu, small):
     got = evaluate(mod, Th, Ph, Yh, gpu)
     assert surrogate.fallback_count(mod) == 4096
     ref, floor = o.batch(Th, Ph, Yh, rates=False, threads=8)
-    assert relative_error(got, ref, floor).max() < 1e-5
+    assert relative_error(got, ref, floor).max() <= 1e-12
 
 
 @pytest.mark.gpu
@@ -116,3 +116,27 @@ def test_trained_network_meets_the_stated_bound(gpu):
     """The stated bound: every held-out output within 1e-3 of the FP64 oracle (fallback states included)."""
     err, _ = held_out(gpu)
     assert err.max() <= SURROGATE_TOL
+
+
+@pytest.mark.gpu
+def test_surrogate_with_the_mfma_mixed_rates(gpu):
+    """The north star's pairing: the network's transport beside the MFMA mixed-precision rates kernel (split-BF16
+    contractions, FP32 exp/log, FP64 ΔG/RT and forward-minus-reverse), on held-out trajectory states: ω within the
+    mixed form's 1e-5 of the state floor and transport within 1e-3 of the FP64 oracle; ω the same bits as the mixed
+    form's own."""
+    if not os.path.exists(TRAINED):
+        pytest.skip("no trained network committed")
+    from helpers import state_floor
+    M, mech, o = setup("gri-53")
+    trained = surrogate.Trained.load(TRAINED)
+    T, P, Y = reactor.trajectories(M, 64, 1 << 13, seed=98, max_steps=600, batch=64)
+    ref, floor = o.batch(T, P, Y, threads=8)
+    prog = M.chemistry()
+    got = evaluate(surrogate.assemble(prog, trained, rates="mixed"), T, P, Y, gpu)
+    mixed = evaluate(prog.assemble(form="mixed"), T, P, Y, gpu)
+    Ka = M.active
+    assert np.array_equal(got[:Ka], mixed[:Ka])
+    err = relative_error(got, ref, state_floor(floor, Ka))
+    print(f"surrogate + mixed rates: ω {err[:Ka].max():.2e}, transport {err[Ka:].max():.2e}", flush=True)
+    assert err[:Ka].max() <= 1e-5 and err[Ka:].max() <= SURROGATE_TOL
+
