@@ -1,9 +1,12 @@
 // Net production rates in FP64 for mechanisms too large for a straight-line kernel (hand-written; the host prepends the
-// mechanism sizes). A wave owns NS = 8 states and splits the reactions over G = 8 lane groups (lane = 8 group + state):
+// mechanism sizes). A wave owns NS states and splits the reactions over G = 64/NS lane groups (lane = NS group +
+// state); NS is the largest power of two up to 8 whose per-state arrays fit about 10 KiB of LDS per wave, so that 16
+// waves share a CU (NS = 8 at 10 species, 4 at 53, 2 at 160):
 //   1. per state, spread over the groups: concentrations C_k (reaction.rs:362-373 on a mass basis), the clamped
 //      exponentials E+_k = min(c+, e^g_k) and E-_k = min(c-, 1/e^g_k) of the Gibbs functions g_k = G°_k/RT of the
 //      state's NASA-7 piece (reaction.rs:236-268), and zeroed production rates, all in LDS ([species][state]);
-//   2. rounds of G reactions, one per group, every reaction of a round of one model (the host sorts and pads them):
+//   2. rounds of G reactions, one per group, every reaction of a round of one model (the host sorts and pads them),
+//      the next round's table rows loaded while the current one computes:
 //      k_f from its Arrhenius exponent exp(ln A + β ln T - θ/T) clamped at FLT_MAX (reaction.rs:113-139), the third-
 //      body sum M = C_tot + Σ (e_k - 1) C_k (reaction.rs:141-145), Lindemann or Troe falloff (reaction.rs:149-193);
 //      q = k_f Π C^ν' - k_f (1/K_c) Π C^ν'' with 1/K_c = Π E±^|ν| (P0/RT)^(-Σν) (reaction.rs:271-308);
@@ -14,12 +17,12 @@
 //   (T1 >= 1e30: no α e^(-T/T1) term, α instead; T2 NaN: no e^(-T2/T) term);
 //   info[row] = (model | kinf_const << 4, first efficiency, efficiency count, 0), model 0 elementary, 1 irreversible,
 //   2 three-body, 3 Lindemann, 4 Troe, 5 padding; reactants/products[row] = up to 3 species (with multiplicity, -1
-//   none); net[row][6] = 16 s + (ν + 4) per species changed, -1 none; eff_species/eff_value = the (e_k - 1) lists;
-//   nasa[k] = the 2 x 7 coefficients of g_k (low piece first), split[k] the switch temperature (+inf: one piece).
-// Required defines: K (species), KA (active species), ROUNDS (rounds of G reactions).
+//   none); net[row] = 6 16-bit entries 16 s + (ν + 4), two per word (low half first), 0xFFFF none;
+//   eff_species/eff_value = the (e_k - 1) lists; nasa[k] = the 2 x 7 coefficients of g_k (low piece first), split[k]
+//   the switch temperature (+inf: one piece).
+// Required defines: K (species), KA (active species), ROUNDS (rounds of G reactions), NS (states per wave: 1, 2, 4, 8).
 
-#define NS 8
-#define G 8
+#define G (64 / NS)
 #define RGAS (1.380649e-23 * 6.02214076e23)
 #define P0_R (101325.0 / RGAS)
 #define FLT_MAX_D 3.4028234663852886e38
@@ -50,13 +53,31 @@ __device__ __forceinline__ double exp64(double x) {
   return __builtin_ldexp(p, (int)(unsigned)__double_as_longlong(t));
 }
 
-// Sum over the G groups of a state (lanes col, col + 8, ..., col + 56).
+// Sum over the G groups of a state (lanes col, col + NS, ..., col + 64 - NS).
 __device__ __forceinline__ double group_sum(double v) {
-  v += __shfl_xor(v, 8);
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
+#pragma unroll
+  for (int m = NS; m < 64; m <<= 1) v += __shfl_xor(v, m);
   return v;
 }
+
+// The table row of one reaction that every round reads.
+struct Row {
+  i32x4 info, re, pr, net;
+  f64x4 rate;
+};
+__device__ __forceinline__ Row load_row(int row, const i32x4* __restrict__ info, const f64x4* __restrict__ rate,
+                                        const i32x4* __restrict__ reactants, const i32x4* __restrict__ products,
+                                        const i32x4* __restrict__ net) {
+  Row r;
+  r.info = info[row];
+  r.rate = rate[row];
+  r.re = reactants[row];
+  r.pr = products[row];
+  r.net = net[row];
+  return r;
+}
+// Entry i (0..5) of a row's net changes: 16 s + (ν + 4), or 0xFFFF.
+__device__ __forceinline__ int net_entry(const i32x4& w, int i) { return (w[i >> 1] >> (16 * (i & 1))) & 0xFFFF; }
 
 struct Lds {
   double C[K][NS];
@@ -65,17 +86,20 @@ struct Lds {
   double W[KA][NS];
 };
 
-extern "C" __global__ void __launch_bounds__(64)
+// at most 128 registers: four waves per SIMD, the 16 per CU that the LDS sizing allows
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n,
             const f64x4* __restrict__ rate, const f64x4* __restrict__ k0, const f64x4* __restrict__ troe,
             const i32x4* __restrict__ info, const i32x4* __restrict__ reactants, const i32x4* __restrict__ products,
-            const int* __restrict__ net, const int* __restrict__ eff_species, const double* __restrict__ eff_value,
+            const i32x4* __restrict__ net, const int* __restrict__ eff_species, const double* __restrict__ eff_value,
             const double* __restrict__ rcp_W, const double* __restrict__ nasa, const double* __restrict__ split,
             double clamp_pos, double clamp_neg) {
   __shared__ Lds L;
-  const int lane = threadIdx.x, col = lane & (NS - 1), grp = lane >> 3;
+  const int lane = threadIdx.x, col = lane & (NS - 1), grp = lane / NS;
   const unsigned s = blockIdx.x * NS + col;
   const bool valid = s < n;
+  // the first round's rows, in flight during the per-state stage
+  Row cur = load_row(grp, info, rate, reactants, products, net);
   // states past n compute on a harmless stand-in and write nothing
   const double T = valid ? in[0][s] : 1000.0, P = valid ? in[1][s] : 101325.0;
   const double rcpT = 1.0 / T, lnT = log(T), T2 = T * T;
@@ -110,14 +134,15 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
   // 2. rounds of G reactions
   for (int j = 0; j < ROUNDS; j++) {
     const int row = G * j + grp;
-    const i32x4 inf = info[row];
-    const int model = inf[0] & 15;  // the round's model, or 5 on the padding rows that end a model's last round
+    const Row r = cur;
+    if (j + 1 < ROUNDS) cur = load_row(row + G, info, rate, reactants, products, net);
+    const int model = r.info[0] & 15;  // the round's model, or 5 on the padding rows that end a model's last round
     if (model == 5) continue;
-    const f64x4 c = rate[row];
+    const f64x4 c = r.rate;
     double kf = fmin(FLT_MAX_D, exp64(c[0] + c[1] * lnT - c[2] * rcpT));
     if (model >= 2) {
       double M = C_tot;
-      for (int e = inf[1]; e < inf[1] + inf[2]; e++) M = __builtin_fma(eff_value[e], L.C[eff_species[e]][col], M);
+      for (int e = r.info[1]; e < r.info[1] + r.info[2]; e++) M = __builtin_fma(eff_value[e], L.C[eff_species[e]][col], M);
       if (model == 2) {
         kf *= M;
       } else {
@@ -126,8 +151,8 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
         if (model == 3) {
           kf = M * k0v * kf / (M * k0v + kf + FLT_MIN_D);  // Lindemann (reaction.rs:153-158)
         } else {
-          // Troe (reaction.rs:160-193): k0 and k∞ with A∞ factored out (kf holds A∞ k∞' unless k∞ is constant)
-          const bool kinf_const = (inf[0] >> 4) & 1;
+          // Troe (reaction.rs:160-193): k0 and k∞ with A∞ factored out
+          const bool kinf_const = (r.info[0] >> 4) & 1;
           const double A_inf = c0[3];
           const double kinf = kinf_const ? 1.0 : fmin(FLT_MAX_D, exp64(c[1] * lnT - c[2] * rcpT));
           const double Pr = kinf_const ? M * k0v : M * k0v / (kinf + FLT_MIN_D);
@@ -143,31 +168,35 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
         }
       }
     }
-    const i32x4 re = reactants[row], pr = products[row];
     double q = kf;
 #pragma unroll
     for (int i = 0; i < 3; i++)
-      if (re[i] >= 0) q *= L.C[re[i]][col];
+      if (r.re[i] >= 0) q *= L.C[r.re[i]][col];
     if (model != 1) {
       double b = kf * (c[3] == 0.0 ? 1.0 : c[3] == 1.0 ? C0 : c[3] == -1.0 ? rcpC0 : rcpC0 * rcpC0);
 #pragma unroll
       for (int i = 0; i < 6; i++) {
-        const int e = net[6 * row + i];
-        if (e >= 0) {
+        const int e = net_entry(r.net, i);
+        if (e != 0xFFFF) {
           const int sp = e >> 4, nu = (e & 15) - 4;
           const double f = nu > 0 ? L.Ep[sp][col] : L.En[sp][col];
-          for (int m = 0; m < (nu > 0 ? nu : -nu); m++) b *= f;
+          const int m = nu > 0 ? nu : -nu;  // 1 to 4
+          double fm = f;
+          if (m >= 2) fm *= f;
+          if (m >= 3) fm *= f;
+          if (m >= 4) fm *= f;
+          b *= fm;
         }
       }
 #pragma unroll
       for (int i = 0; i < 3; i++)
-        if (pr[i] >= 0) b *= L.C[pr[i]][col];
+        if (r.pr[i] >= 0) b *= L.C[r.pr[i]][col];
       q -= b;
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-      const int e = net[6 * row + i];
-      if (e >= 0) atomicAdd(&L.W[e >> 4][col], (double)((e & 15) - 4) * q);
+      const int e = net_entry(r.net, i);
+      if (e != 0xFFFF) atomicAdd(&L.W[e >> 4][col], (double)((e & 15) - 4) * q);
     }
   }
   __syncthreads();

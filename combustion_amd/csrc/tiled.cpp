@@ -206,9 +206,9 @@ std::string rates_mixed_text(const MixedTables& t) {
 }
 
 // Tables of kernels/rates_tiled.hip (layouts described there): rows sorted by model, each model's rows padded to whole
-// rounds of 8 so that a round is of one model.
+// rounds of G = 64/NS so that a round is of one model.
 struct TiledRatesTables {
-  int K, KA, ROUNDS;
+  int K, KA, ROUNDS, NS;
   std::vector<double> rate, k0, troe, eff_value, rcp_W, nasa, split;
   std::vector<int32_t> info, reactants, products, net, eff_species;
 };
@@ -218,7 +218,12 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
   const Species& sp = m.species;
   t.K = (int)sp.len();
   t.KA = (int)m.active;
-  const int R = (int)m.reactions.size(), G = 8;
+  // states per wave: the largest power of two up to 8 whose LDS arrays (C, E+, E-, ω) fit 10 KiB, for 16 waves per CU
+  const size_t per_state = ((size_t)t.K + 3 * (size_t)t.KA) * 8;
+  t.NS = 8;
+  while (t.NS > 1 && t.NS * per_state > 10240) t.NS /= 2;
+  if (t.NS * per_state > 64 * 1024) throw Unsupported("mechanism too large for the tiled FP64 rates kernel's LDS");
+  const int R = (int)m.reactions.size(), G = 64 / t.NS;
   auto model_code = [](const Reaction& x) {
     if (!x.plog.empty()) throw Unsupported("the tiled FP64 rates kernel has no pressure-dependent Arrhenius tables (the straight-line forms do)");
     switch (x.model) {
@@ -248,7 +253,7 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
   t.info.assign(4 * RP, 0);
   t.reactants.assign(4 * RP, -1);
   t.products.assign(4 * RP, -1);
-  t.net.assign(6 * RP, -1);
+  t.net.assign(4 * RP, -1);  // 0xFFFF halves: no entry
   for (size_t r = 0; r < RP; r++) {
     if (rows[r] < 0) {
       t.info[4 * r] = 5;
@@ -303,7 +308,10 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
       const int nu = x.net[s];
       if (!nu) continue;
       if (n == 6 || std::abs(nu) > 4) throw Unsupported("too many or too large stoichiometric changes in reaction " + x.equation);
-      t.net[6 * r + n++] = (int32_t)s * 16 + (nu + 4);
+      uint32_t& word = reinterpret_cast<uint32_t&>(t.net[4 * r + n / 2]);
+      const uint32_t entry = (uint32_t)s * 16 + (uint32_t)(nu + 4), shift = 16 * (n % 2);
+      word = (word & ~(0xFFFFu << shift)) | (entry << shift);
+      n++;
     }
   }
   if (t.eff_species.empty()) {  // keep the device arrays non-empty
@@ -320,14 +328,13 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
     }
     t.split.push_back(std::isnan(nasa.temperature_split) ? INFINITY : nasa.temperature_split);
   }
-  const size_t lds = ((size_t)t.K + 3 * (size_t)t.KA) * 8 * 8;
-  if (lds > 64 * 1024) throw Unsupported("mechanism too large for the tiled FP64 rates kernel's LDS");
   return t;
 }
 
 std::string rates_tiled_text(const TiledRatesTables& t) {
   std::ostringstream s;
-  s << "#define K " << t.K << "\n#define KA " << t.KA << "\n#define ROUNDS " << t.ROUNDS << "\n" << rates_tiled_source;
+  s << "#define K " << t.K << "\n#define KA " << t.KA << "\n#define ROUNDS " << t.ROUNDS << "\n#define NS " << t.NS << "\n"
+    << rates_tiled_source;
   return s.str();
 }
 
@@ -512,7 +519,7 @@ class RatesTiled {
     double pos = 6981463048192., neg = (double)cbrtf(3.40282347e38f);  // reaction.rs:244-256
     void* args[] = {&in, &out, &n32, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_reactants_, &d_products_, &d_net_, &d_eff_species_,
                     &d_eff_value_, &d_rcp_W_, &d_nasa_, &d_split_, &pos, &neg};
-    const unsigned states_per_block = 8;
+    const unsigned states_per_block = (unsigned)t_.NS;
     device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + states_per_block - 1) / states_per_block), 1, 1, 64, 1, 1, 0,
                                         stream, args, nullptr),
                   "launch rates_tiled");

@@ -32,14 +32,12 @@ def sections(out, Ka, rates):
 @pytest.mark.parametrize("name,n,flags", [("h2o2-10", 4096, 3), ("gri-53", 4096, 3), ("large-160", 2048, 3)])
 @pytest.mark.parametrize("form", ["straight", "fp64"])
 def test_fp64_matches_oracle(name, n, flags, form, gpu):
-    """Straight-line program (form 0) and the tiled FP64 form (straight rates, MFMA transport) to 1e-12."""
+    """Straight-line program (form 0) and the FP64 form to 1e-12. At K=160 the FP64 form takes the table-driven
+    rates kernel (rates_tiled) and the MFMA transport kernel (transport_tiled on v_mfma_f64_16x16x4f64); the
+    straight-line rates kernel, which a run of this case took when it faulted the GPU during assembly, is refused
+    above 64 species."""
     if name == "large-160" and form == "straight":
         pytest.skip("the whole K=160 program as one straight-line kernel is not compiled (the fp64 form covers K=160)")
-    if name == "large-160" and form == "fp64":
-        pytest.skip("a run of this case, when it took the straight-line rates kernel (now refused above 64 species) and "
-                    "an earlier transport kernel, faulted the GPU (memory aperture violation) for a cause not established; "
-                    "it stays off the GPU, and test_fp64_at_160_species covers K=160 in FP64 with the kernels that serve it "
-                    "now")
     M, mech, o = setup(name)
     T, P, Y = synth.config_states(name, n, 7)
     rates = bool(flags & 1)
@@ -48,21 +46,6 @@ def test_fp64_matches_oracle(name, n, flags, form, gpu):
     assert np.all(np.isfinite(out))
     err = relative_error(out, ref, floor)
     print(name, form, "max error per section:", {k: float(v.max()) for k, v in sections(err, mech["Ka"], rates).items()}, flush=True)
-    assert err.max() <= TOL
-
-
-@pytest.mark.gpu
-def test_fp64_at_160_species(gpu):
-    """K=160 in the FP64 form: the table-driven rates kernel (rates_tiled) and the MFMA transport kernel
-    (transport_tiled on v_mfma_f64_16x16x4f64), to 1e-12 of the oracle."""
-    M, mech, o = setup("large-160")
-    T, P, Y = synth.config_states("large-160", 2048, 7)
-    ref, floor = o.batch(T, P, Y, threads=8)
-    out = run(M.chemistry(3), "fp64", T, P, Y, gpu)
-    assert np.all(np.isfinite(out))
-    err = relative_error(out, ref, floor)
-    print("large-160 fp64 max error per section:", {k: float(v.max()) for k, v in sections(err, mech["Ka"], True).items()},
-          flush=True)
     assert err.max() <= TOL
 
 

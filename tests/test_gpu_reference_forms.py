@@ -163,11 +163,11 @@ def test_tiled_rates_at_small_species_counts(spec, gpu):
     Troe with and without a constant k∞): to 1e-12 of the oracle."""
     if spec == "handwritten":
         M, mech, o = setup_text(open(os.path.join(ROOT, "tests", "golden", "handwritten.yaml")).read())
-        T, P, Y = synth.states(M.K, 2048, 13, (300.0, 3000.0), (0.1, 50.0), 0.5)
+        T, P, Y = synth.states(M.K, 2045, 13, (300.0, 3000.0), (0.1, 50.0), 0.5)
     else:
         variant, _, name = spec.rpartition(":")
         M, mech, o = setup_text(models_variant(name)) if variant else setup(name)
-        T, P, Y = synth.config_states(name, 4096, 76)
+        T, P, Y = synth.config_states(name, 4093, 76)  # a ragged count: the last wave's states are partly absent
     ref, floor = o.batch(T, P, Y, threads=8)
     src = M.chemistry().hip_source("fp64+tiled-rates")
     assert "rates_tiled" in src and "rates_fp64" not in src
