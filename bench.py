@@ -8,9 +8,12 @@ FP64 oracle on a sample of the same states, and — when a trained network is pr
 2^20 held-out reactor-trajectory states, FP64 against FP32-core rates + network transport. Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--states N] [--no-cpu] [--surrogate PATH]
-  python bench.py --only rates|transport|fp64|mixed|fast|surrogate|copy [--steps K]   (one launch kind alone, for
-      rocprofv3 runs; surrogate runs the surrogate form on 2^20 held-out trajectory states; copy is a device-to-device
-      copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
+  python bench.py --only rates|transport|fp64|mixed|fast|surrogate|copy [--steps K] [--mechanism large-160]
+      (one launch kind alone, for rocprofv3 runs; surrogate runs the surrogate form on 2^20 held-out trajectory
+      states; copy is a device-to-device copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
+
+The line also carries the 160-species / 1800-reaction workload (FP64, mixed and fast forms; --no-large leaves it
+out).
 """
 import argparse
 import json
@@ -26,6 +29,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 MECHANISM = "gri-53"
+LARGE = "large-160"
 SURROGATE = os.path.join(ROOT, "combustion_amd", "data", "surrogate_gri53.npz")
 
 
@@ -109,12 +113,12 @@ def time_module(mod, inputs, outputs, steps, warmup):
     return wall, float(np.mean(kernel_ms))
 
 
-def errors_vs_oracle(out_dev, T, P, Y, n_sample, Ka, rates=True, exact=False):
+def errors_vs_oracle(out_dev, T, P, Y, n_sample, Ka, rates=True, exact=False, mechanism=MECHANISM):
     """Per-section max and median relative error of a form's outputs on the first n_sample states: against the
     term-scale floor of the FP64 forms (exact) or the mixed forms' state floor."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import relative_error, setup, state_floor
-    _, _, o = setup(MECHANISM)
+    _, _, o = setup(mechanism)
     ref, floor = o.batch(T[:n_sample], P[:n_sample], Y[:, :n_sample], rates=rates, threads=16)
     got = np.stack([x[:n_sample].cpu().numpy() for x in out_dev])
     f = floor if exact else state_floor(floor, Ka if rates else 0)
@@ -158,6 +162,48 @@ def surrogate_workload(args, M, prog, dev):
     return out
 
 
+def large_mechanism_workload(args, dev):
+    """Config 3: the 160-species / 1800-reaction mechanism at 2^20 states. The FP64 form (table-driven rates kernel,
+    transport_tiled on v_mfma_f64_16x16x4f64), its two kernels alone, and the mixed forms, each with its error
+    against the FP64 oracle on a sample of the same states."""
+    import torch
+    from combustion_amd import Mechanism, synth
+    M = Mechanism(synth.mechanism_path(LARGE))
+    N = args.states
+    T, P, Y = synth.config_states(LARGE, N, 2025)
+    inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
+    prog = M.chemistry()
+    outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
+    steps = max(3, args.steps // 4)
+    sample = max(256, args.error_sample // 4)
+    out = {"workload": f"rates+transport, K={M.K} species / R={M.R} reactions, {N} states", "steps": steps,
+           "error_sample": sample}
+    wall64 = None
+    for form in ("fp64", "mixed", "fast"):
+        mod = prog.assemble(form=form)
+        wall, ms = time_module(mod, inputs, outputs, steps, 1)
+        entry = {"states_per_s": N * steps / wall, "kernel_ms": ms,
+                 "error_vs_oracle": errors_vs_oracle(outputs, T, P, Y, sample, M.active, exact=form == "fp64", mechanism=LARGE)}
+        if form == "fp64":
+            wall64 = wall
+            sections = {}
+            for name, flags, kernel in (("rates", 1, "rates_tiled"), ("transport", 2, "transport_tiled")):
+                p = M.chemistry(flags)
+                m = p.assemble(form="fp64")
+                outs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(p.n_outputs)]
+                _, kms = time_module(m, inputs, outs, steps, 1)
+                flop = p.op_counts()["weighted"]
+                sections[name] = {"kernel": kernel, "kernel_ms": kms, "flop_per_state": flop,
+                                  "achieved_tflops": flop * N / (kms * 1e-3) / 1e12}
+                del m, outs
+            entry["sections"] = sections
+        else:
+            entry["speedup_vs_fp64"] = wall64 / wall
+        out[form] = entry
+        del mod
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +215,8 @@ def main():
     ap.add_argument("--error-sample", type=int, default=8192)
     ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "fast", "surrogate", "copy"])
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--mechanism", default=MECHANISM, help="for --only: gri-53 (default) or large-160")
+    ap.add_argument("--no-large", action="store_true", help="leave out the 160-species workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every state is independent, so N GPUs would each run "
@@ -182,10 +230,11 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
-    M = Mechanism(synth.mechanism_path(MECHANISM))
+    name = args.mechanism if args.only else MECHANISM
+    M = Mechanism(synth.mechanism_path(name))
     Ka = M.active
     N = args.states
-    T, P, Y = synth.config_states(MECHANISM, N, 2024)
+    T, P, Y = synth.config_states(name, N, 2024)
     inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
     prog = M.chemistry()
     outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
@@ -260,6 +309,11 @@ def main():
     if os.path.exists(args.surrogate):
         result["surrogate_workload"] = surrogate_workload(args, M, prog, dev)
     result["forms"] = forms
+    del inputs, outputs, fp64
+    torch.cuda.empty_cache()
+    if not args.no_large:
+        log("160-species workload")
+        result["large_mechanism"] = large_mechanism_workload(args, dev)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(prog, MECHANISM)
     print(json.dumps(result))
@@ -282,7 +336,7 @@ def only(args, M, inputs, N):
     mod = p.assemble(form=args.only if args.only in ("mixed", "fast") else "fp64")
     outs = [torch.empty(N, dtype=torch.float64, device=inputs[0].device) for _ in range(p.n_outputs)]
     wall, ms = time_module(mod, inputs, outs, args.steps, 1)
-    print(json.dumps({"only": args.only, "states": N, "steps": args.steps, "kernel_ms": ms,
+    print(json.dumps({"only": args.only, "mechanism": args.mechanism, "states": N, "steps": args.steps, "kernel_ms": ms,
                       "flop_per_state": p.op_counts()["weighted"],
                       "algorithmic_hbm_bytes_per_launch": 8 * N * (p.n_inputs + p.n_outputs)}))
 

@@ -96,7 +96,12 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
             double clamp_pos, double clamp_neg) {
   __shared__ Lds L;
   const int lane = threadIdx.x, col = lane & (NS - 1), grp = lane / NS;
-  const unsigned s = blockIdx.x * NS + col;
+  // XCD-aware state ranges: consecutive blocks go to the 8 XCDs in turn, so block b takes state group
+  // xcd·per + min(xcd, rem) + b/8 (a bijection of [0, gridDim.x)), and the blocks of one XCD cover one contiguous
+  // range: the NS·8 B that a block reads of each 128 B line come from its own XCD's L2, not once per XCD from HBM
+  const unsigned nb = gridDim.x, per = nb / 8, rem = nb % 8, xcd = blockIdx.x % 8;
+  const unsigned group = xcd * per + min(xcd, rem) + blockIdx.x / 8;
+  const unsigned s = group * NS + col;
   const bool valid = s < n;
   // the first round's rows, in flight during the per-state stage
   Row cur = load_row(grp, info, rate, reactants, products, net);
