@@ -1,7 +1,8 @@
 """ORACLE — test infrastructure only.
 
-CPU restatement of the reference's per-state chemistry, used as the checker of the HIP path and as the CPU
-baseline. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import it; the product
+CPU restatement of the reference's per-state chemistry, used as the checker of the HIP path (and timed, for
+information, beside the CPU baseline, which is the product's own host-compiled program). Only tests/,
+__graft_entry__.smoke() and bench.py (its error samples and cpu_baseline leg) may import it; the product
 (combustion_amd) never does.
 
   mechanism.py  Cantera-YAML front end (yaml.rs:40-171, lib.rs:22-140)
