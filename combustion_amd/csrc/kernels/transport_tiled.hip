@@ -9,10 +9,14 @@
 //       FP32), species × states, one 16-row tile of k per wave at a time
 //   ρD_k = √T (Σ_j W_j x_j - W_k x_k) / (R S_k),  S_k = Σ_{j≠k} x_j / d_kj(ℓ)     (transport.rs:162-177)
 //       d_kj for j < k on MFMA: (16 pairs × 4 coefficients)·(ℓ⁰..ℓ³ × 16 states) per tile, a tile being 16
-//       consecutive j of one k; 1/d on VALU; S_k by a cross-lane sum and S_j by an LDS update, into per-wave
-//       partial sums that are added in wave order at the end (the same order for every state).
+//       consecutive j of one k; 1/d on VALU. Row by row (k = 1..K-1), the waves split row k's tiles by j block
+//       (block b always to wave b mod WAVES): each adds x_k/d_kj into S_j in LDS (no other wave writes S_j) and sums
+//       x_j/d_kj across its lanes; after a barrier the owner of k's block adds the waves' row sums, in wave order,
+//       into S_k. One S array serves the block (it reuses the 1/v_k array of the viscosity stage), and every state
+//       adds the terms in the same order.
 // Outputs λ [W/m/K], μ [Pa s], D_k = ρD_k/ρ [m²/s] with ρ = Σ W_k x_k.
-// Required defines: K, KP (K rounded up to 16), NTILE (diffusion tiles), DK (fit length rounded up to 4: 4 or 8),
+// Required defines: K, KP (K rounded up to 16), NTILE (diffusion tiles, K-1 rows of ⌈k/16⌉), DK (fit length
+// rounded up to 4: 4 or 8),
 // F64 (1: FP64 arithmetic and v_mfma_f64_16x16x4f64; 0: FP32 arithmetic and v_mfma_f32_16x16x4f32).
 // With a state list (idx, count) the kernel evaluates only the listed states, in place.
 
@@ -46,11 +50,14 @@ __device__ __forceinline__ real sqrt_real(double x) { return __builtin_sqrtf((fl
 #define KG ((K + 15) / 16)  // species per lane group (16 groups: 4 per wave)
 
 struct Lds {
-  real x[KP][NS];          // concentrations (0 past K)
-  real r[KP][NS];          // 1 / v_k (1 past K)
-  real S[WAVES][KP][NS];   // per-wave partial Σ_{j≠k} x_j / d_kj
-  real pw[DK][NS];         // ℓ^d
-  double red[WAVES][4][NS];  // per-wave partial reductions
+  real x[KP][NS];  // concentrations (0 past K)
+  union {
+    real r[KP][NS];  // 1 / v_k (1 past K), in the viscosity stage
+    real S[KP][NS];  // Σ_{j≠k} x_j / d_kj, in the diffusion stage
+  };
+  real pw[DK][NS];                 // ℓ^d
+  real part[2][WAVES][NS];         // the waves' sums over one row of the diffusion stage (by row parity)
+  double red[WAVES][4][NS];        // per-wave partial reductions
 };
 
 // Sum over the 4 lane groups of a wave (lanes col, col+16, col+32, col+48), then over the waves in order.
@@ -64,11 +71,12 @@ __device__ __forceinline__ double block_sum(Lds& L, int slot, double v, int wave
   return s;
 }
 
-extern "C" __global__ void __launch_bounds__(64 * WAVES)
+// three waves per SIMD (at most 168 registers), which the LDS of K = 160 (44 KiB per block) also allows
+extern "C" __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
 transport_tiled(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n,
                 const double* __restrict__ rcp_W, const double* __restrict__ W, const real* __restrict__ fit_c,
                 const real* __restrict__ fit_v, const real* __restrict__ wilke, const real* __restrict__ dcoef,
-                const int* __restrict__ tiles, double T0, const unsigned* __restrict__ idx, const unsigned* __restrict__ count) {
+                double T0, const unsigned* __restrict__ idx, const unsigned* __restrict__ count) {
   __shared__ Lds L;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4, g16 = 4 * wave + grp;
@@ -132,9 +140,8 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
     }
     L.x[k][col] = x;
     L.r[k][col] = r;
-    for (int w = 0; w < WAVES; w++) L.S[w][k][col] = 0;
   }
-  // (block_sum's barrier also publishes x, r, S and pw)
+  // (block_sum's barrier also publishes x, r and pw)
   A = block_sum(L, 2, A, wave, grp, col);
   B = block_sum(L, 3, B, wave, grp, col);
   WN = block_sum(L, 0, WN, wave, grp, col);
@@ -163,30 +170,47 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
     }
   }
 
-  // 3. Binary diffusion: tile t = (k, j0) gives rows j = j0 + ROW(grp, i).
+  // 3. Binary diffusion: S (over the 1/v_k array) cleared once every wave is past the viscosity stage; tile
+  //    t = (k, j0) gives rows j = j0 + ROW(grp, i); the tiles of row k start at t0 = Σ_{k' < k} ⌈k'/16⌉.
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < KG; m++) L.S[g16 + 16 * m][col] = 0;
+  __syncthreads();
   const real lp0 = L.pw[grp][col];
   const real lp1 = DK > 4 ? L.pw[(4 + grp) % DK][col] : (real)0;
-  real (*S)[NS] = L.S[wave];
-  for (int t = wave; t < NTILE; t += WAVES) {
-    const int k = tiles[2 * t], j0 = tiles[2 * t + 1];
-    r4 d = {0, 0, 0, 0};
-    d = MFMA(dcoef[(size_t)t * (DK / 4) * 64 + lane], lp0, d);
-    if (DK > 4) d = MFMA(dcoef[((size_t)t * (DK / 4) + 1) * 64 + lane], lp1, d);
+  int t0 = 0;
+  for (int k = 1; k < K; k++) {
+    const int nb = (k + 15) / 16;
     const real xk = L.x[k][col];
     real sk = 0;
+    for (int jb = wave; jb < nb; jb += WAVES) {
+      const int t = t0 + jb, j0 = 16 * jb;
+      r4 d = {0, 0, 0, 0};
+      d = MFMA(dcoef[(size_t)t * (DK / 4) * 64 + lane], lp0, d);
+      if (DK > 4) d = MFMA(dcoef[((size_t)t * (DK / 4) + 1) * 64 + lane], lp1, d);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int j = j0 + ROW(grp, i);
-      if (j < k) {
-        const real R = rcp(d[i]);
-        sk += L.x[j][col] * R;
-        S[j][col] += xk * R;
+      for (int i = 0; i < 4; i++) {
+        const int j = j0 + ROW(grp, i);
+        if (j < k) {
+          const real R = rcp(d[i]);
+          sk += L.x[j][col] * R;
+          L.S[j][col] += xk * R;
+        }
       }
     }
     sk += __shfl_xor(sk, 16);
     sk += __shfl_xor(sk, 32);
-    if (grp == 0) S[k][col] += sk;
-    __builtin_amdgcn_wave_barrier();
+    if (grp == 0) L.part[k & 1][wave][col] = sk;
+    __syncthreads();
+    // the wave that owns k's block (and so every other update of S_k) adds the row's sums; the next row's sums go
+    // to the other buffer, and this buffer is rewritten only after the next barrier, which this wave reaches later
+    if (wave == (k / 16) % WAVES && grp == 0) {
+      real sum = L.part[k & 1][0][col];
+#pragma unroll
+      for (int w = 1; w < WAVES; w++) sum += L.part[k & 1][w][col];
+      L.S[k][col] += sum;
+    }
+    t0 += nb;
   }
   const double mu_all = block_sum(L, 1, (double)mu, wave, grp, col);
 
@@ -200,10 +224,6 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
 #pragma unroll
   for (int m = 0; m < KG; m++) {
     const int k = g16 + 16 * m;
-    if (k < K) {
-      double Sk = (double)L.S[0][k][col];
-      for (int w = 1; w < WAVES; w++) Sk += (double)L.S[w][k][col];
-      out[2 + k][s] = sqrtT * (WN - W[k] * (double)L.x[k][col]) / (RGAS * Sk) / WN;
-    }
+    if (k < K) out[2 + k][s] = sqrtT * (WN - W[k] * (double)L.x[k][col]) / (RGAS * (double)L.S[k][col]) / WN;
   }
 }

@@ -344,7 +344,6 @@ struct TransportTables {
   int K, KP, DK, NTILE;
   double T0;
   std::vector<double> rcp_W, W, fit_c, fit_v, wilke, dcoef;
-  std::vector<int32_t> tiles;  // (k, j0) per diffusion tile
 };
 
 TransportTables transport_tables(const Mechanism& m, const Polynomials& p, double T0) {
@@ -380,18 +379,18 @@ TransportTables transport_tables(const Mechanism& m, const Polynomials& p, doubl
         t.wilke[at + 64] = a * b;
         t.wilke[at + 128] = b * b;
       }
-  // Diffusion tiles: 16 consecutive j < k for one k; A[m][d] = coefficient d of d_{k, j0+m} (zero past j = k-1).
+  // Diffusion tiles: 16 consecutive j < k for one k, rows k = 1..K-1 in order and j0 ascending within a row (the
+  // kernel finds row k's tiles at Σ_{k' < k} ⌈k'/16⌉); A[m][d] = coefficient d of d_{k, j0+m} (zero past j = k-1).
+  t.NTILE = 0;
   for (int k = 1; k < K; k++)
     for (int j0 = 0; j0 < k; j0 += 16) {
-      t.tiles.push_back(k);
-      t.tiles.push_back(j0);
+      t.NTILE++;
       for (int blk = 0; blk < DK / 4; blk++)
         for (int L = 0; L < 64; L++) {
           const int j = j0 + L % 16, d = 4 * blk + L / 16;
           t.dcoef.push_back(j < k && d < D ? p.diffusivityITVT[((size_t)k * K + j) * D + d] : 0.);
         }
     }
-  t.NTILE = (int)t.tiles.size() / 2;
   return t;
 }
 
@@ -417,7 +416,6 @@ class TransportTiled {
     kernel_.load(device::compile(transport_text(t_, f64), "transport_tiled", hip_options(true)), "transport_tiled");
     d_rcp_W_ = upload(t_.rcp_W);
     d_W_ = upload(t_.W);
-    d_tiles_ = upload(t_.tiles);
     if (f64) {
       d_c_ = upload(t_.fit_c);
       d_v_ = upload(t_.fit_v);
@@ -431,7 +429,7 @@ class TransportTiled {
     }
   }
   ~TransportTiled() {
-    for (void* ptr : {(void*)d_rcp_W_, (void*)d_W_, (void*)d_tiles_, d_c_, d_v_, d_wilke_, d_dcoef_})
+    for (void* ptr : {(void*)d_rcp_W_, (void*)d_W_, d_c_, d_v_, d_wilke_, d_dcoef_})
       if (ptr) (void)hipFree(ptr);
   }
   // Every state, or (with idx and count) the count states listed in idx.
@@ -439,7 +437,7 @@ class TransportTiled {
               const unsigned* count = nullptr) {
     unsigned n32 = (unsigned)n;
     double T0 = t_.T0;
-    void* args[] = {&in, &out, &n32, &d_rcp_W_, &d_W_, &d_c_, &d_v_, &d_wilke_, &d_dcoef_, &d_tiles_, &T0, &idx, &count};
+    void* args[] = {&in, &out, &n32, &d_rcp_W_, &d_W_, &d_c_, &d_v_, &d_wilke_, &d_dcoef_, &T0, &idx, &count};
     const unsigned states_per_block = 16;  // 4 waves share 16 states
     unsigned grid = (unsigned)((n + states_per_block - 1) / states_per_block);
     device::check(hipModuleLaunchKernel(kernel_.function, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), "launch transport");
@@ -450,7 +448,6 @@ class TransportTiled {
   bool f64_;
   device::Kernel kernel_;
   double *d_rcp_W_ = nullptr, *d_W_ = nullptr;
-  int32_t* d_tiles_ = nullptr;
   void *d_c_ = nullptr, *d_v_ = nullptr, *d_wilke_ = nullptr, *d_dcoef_ = nullptr;
 };
 
