@@ -13,7 +13,7 @@ FP64 oracle on a sample of the same states, and — when a trained network is pr
       states; copy is a device-to-device copy of --copy-mib MiB, the reference for the FETCH_SIZE/WRITE_SIZE counters)
 
 The line also carries the 160-species / 1800-reaction workload (FP64, mixed and fast forms; --no-large leaves it
-out).
+out) and the batched reactor integrator on 2^16 H2/O2 reactors (--no-integrator leaves it out).
 """
 import argparse
 import json
@@ -204,6 +204,56 @@ def large_mechanism_workload(args, dev):
     return out
 
 
+def integrator_workload(args, dev):
+    """the batched reactor integrator (cb_integrator_*, Rosenbrock ode23s with the analytic Jacobian)
+    on 2^16 constant-pressure reactors of the hand-written H2/O2 mechanism (tests/golden/handwritten.yaml: the twelve
+    initial states of tests/golden/reactor_handwritten.npz repeated with T0 scaled by U[0.95, 1.05]), integrated
+    through ignition to 1 ms at rtol 1e-8. Work per reactor step: two right-hand-side evaluations, one Jacobian
+    program (its own right-hand side included), the LU of the (K+1)² matrix and three solves."""
+    import torch
+    from combustion_amd import Mechanism
+    from combustion_amd.native import Integrator
+    golden = os.path.join(ROOT, "tests", "golden")
+    d = np.load(os.path.join(golden, "reactor_handwritten.npz"))
+    M = Mechanism(os.path.join(golden, "handwritten.yaml"))
+    n = 1 << 16
+    reps = -(-n // len(d["T0"]))
+    rng = np.random.default_rng(16)
+    T0 = np.tile(d["T0"], reps)[:n] * rng.uniform(0.95, 1.05, n)
+    Y0 = np.tile(d["Y0"], (1, reps))[:, :n]
+    P = np.tile(d["P"], reps)[:n]
+    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12)
+
+    def run(t_end):
+        y = torch.from_numpy(np.vstack([T0, Y0])).to(dev).contiguous()
+        Pd = torch.from_numpy(P).to(dev)
+        t = torch.zeros(n, dtype=torch.float64, device=dev)
+        h = torch.full((n,), 1e-9, dtype=torch.float64, device=dev)
+        status = torch.zeros(n, dtype=torch.int32, device=dev)
+        steps = torch.zeros(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        taken = integ.advance(y, Pd, t, h, status, steps, t_end, max_steps=200000)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, taken, status, steps, y
+
+    run(1e-5)  # warm-up: module loads
+    wall, taken, status, steps, y = run(1e-3)
+    m = M.K + 1
+    rhs = M.reactor()
+    jac = rhs.jacobian([0] + [2 + k for k in range(M.K)])
+    flop_step = 2 * rhs.op_counts()["weighted"] + jac.op_counts()["weighted"] + (2 * m ** 3) // 3 + 3 * 2 * m * m
+    reactor_steps = int(steps.sum().item())
+    return {"workload": f"{n} constant-pressure H2/O2 reactors (hand-written mechanism, K={M.K}), 0 -> 1 ms through "
+                        f"ignition, rtol 1e-8", "reactors": n, "batch_steps": taken, "wall_ms": wall * 1e3,
+            "reactor_steps": reactor_steps, "reactor_steps_per_s": reactor_steps / wall,
+            "steps_per_reactor": {"min": int(steps.min().item()), "max": int(steps.max().item())},
+            "reached_t_end": float((status == 1).float().mean().item()),
+            "ignited": float((y[0] > torch.from_numpy(T0).to(dev) + 800).float().mean().item()),
+            "flop_per_reactor_step": flop_step,
+            "achieved_tflops": flop_step * reactor_steps / wall / 1e12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,10 +263,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--surrogate", default=SURROGATE)
     ap.add_argument("--error-sample", type=int, default=8192)
-    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "fast", "surrogate", "copy"])
+    ap.add_argument("--only", choices=["rates", "transport", "fp64", "mixed", "fast", "surrogate", "copy", "integrator"])
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--mechanism", default=MECHANISM, help="for --only: gri-53 (default) or large-160")
     ap.add_argument("--no-large", action="store_true", help="leave out the 160-species workload")
+    ap.add_argument("--no-integrator", action="store_true", help="leave out the batched reactor integrator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every state is independent, so N GPUs would each run "
@@ -238,6 +289,9 @@ def main():
     inputs = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in [T, P, *Y]]
     prog = M.chemistry()
     outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
+    if args.only == "integrator":
+        print(json.dumps({"only": "integrator", **integrator_workload(args, dev)}))
+        return
     if args.only == "surrogate":
         from combustion_amd import reactor
         T, P, Y = reactor.trajectories(M, 2048, N, seed=4242, max_steps=800, batch=2048)
@@ -314,6 +368,9 @@ def main():
     if not args.no_large:
         log("160-species workload")
         result["large_mechanism"] = large_mechanism_workload(args, dev)
+    if not args.no_integrator:
+        log("batched reactor integrator")
+        result["integrator"] = integrator_workload(args, dev)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(prog, MECHANISM)
     print(json.dumps(result))
