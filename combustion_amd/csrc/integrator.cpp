@@ -124,13 +124,16 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
   unsigned n32 = (unsigned)n;
+  const unsigned systems_per_wave = 64u / (unsigned)m_;  // kernels/rosenbrock.hip packs ⌊64/M⌋ systems per wave
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
   long taken = 0;
   for (int step = 0; step < max_steps; step++) {
     jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
     for (int phase = 1; phase <= 3; phase++) {
       void* args[] = {&phase, &n32, &jac_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
-      device::check(hipModuleLaunchKernel(step_.function, n32, 1, 1, 64, 1, 1, 0, stream, args, nullptr), "launch rosenbrock");
+      device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,
+                                          nullptr),
+                    "launch rosenbrock");
       if (phase < 3) rhs_->launch_only((const double* const*)tab_ys_, (double* const*)tab_F_, n, stream, nullptr);
     }
     taken++;

@@ -6,16 +6,20 @@
 //   phase 3 (after F2 = f(ys)):    k3 = W⁻¹ (F2 - e32 (k2 - F1) - 2 (k1 - f(y))); error = h (k1 - 2 k2 + k3) / 6;
 //                                   accept (y = ys, t += h) when max_i |error_i| / (atol_i + rtol max(|y_i|, |ys_i|)) <= 1,
 //                                   and the next h from that ratio, clipped to the end time
-// with d = 1/(2+√2) and e32 = 6+√2. A wave owns one system: lane i holds row i of W (M doubles in registers); the
-// pivot row of each elimination step is broadcast with readlane (the pivot lane is wave-uniform), and the LU factors
-// and pivot order are kept in global memory for phases 2 and 3. Systems whose status is not 0 (running) are skipped.
+// with d = 1/(2+√2) and e32 = 6+√2. A wave holds P = ⌊64/M⌋ systems, M lanes each: lane M p + i holds row i of
+// system p's W (M doubles in registers). With one system per wave the pivot row of each elimination step is
+// wave-uniform and broadcast with readlane; with several, every lane reads its system's pivot row with a cross-lane
+// permute (ds_bpermute), the pivot search and the error norm loop over the system's M lanes, and lanes whose system
+// is finished or past the batch compute along without storing. The LU factors and pivot order are kept in global
+// memory for phases 2 and 3. Systems whose status is not 0 (running) are left alone.
 // Layouts: vectors are SoA, component i of system s at [i * n + s]; jac holds f(y) in rows 0..M-1 and J row-major in
 // rows M..M+M²-1 (the Jacobian program's outputs); lu[s][i][j]; perm[s][k] = the row chosen at step k.
-// Required define: M.
+// Required define: M. Launch: ⌈n / P⌉ blocks of 64 threads.
 
 #define D_ROS 0.29289321881345248   // 1 / (2 + √2)
 #define E32 7.4142135623730950      // 6 + √2
 #define H_MIN 1e-30
+#define P (64 / M)                  // systems per wave
 
 // The value a lane holds, for a wave-uniform lane index (two v_readlane_b32).
 __device__ __forceinline__ double lane_value(double v, int lane) {
@@ -25,30 +29,49 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
   return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The value that row r of this lane's system holds (r uniform over the wave when P = 1; base = the system's first
+// lane). Every lane of the wave takes part.
+__device__ __forceinline__ double row_value(double v, int r, int base) {
+#if P == 1
+  (void)base;
+  return lane_value(v, r);
+#else
+  return __shfl(v, base + r);
+#endif
+}
+
 // Solves W x = b with the factors in registers: lane i holds row i (a[k] the multiplier l_ik for k < pos, U_i[k] for
 // k >= pos), pos the step at which row i was the pivot, perm[k] the pivot row of step k. Lane i returns x_i.
-__device__ __forceinline__ double solve(const double (&a)[M], int pos, const int (&perm)[M], double b, int lane) {
+__device__ __forceinline__ double solve(const double (&a)[M], int pos, const int (&perm)[M], double b, int i, int base) {
 #pragma unroll
   for (int k = 0; k < M; k++) {  // forward: L z = P b (unit lower triangular)
-    const int p = perm[k];
-    const double z = lane_value(b, p);
+    const double z = row_value(b, perm[k], base);
     if (pos > k) b = __builtin_fma(-a[k], z, b);
   }
   double x = 0.0;
 #pragma unroll
   for (int k = M - 1; k >= 0; k--) {  // backward: U x = z
     const int p = perm[k];
-    const double xk = lane_value(b, p) / lane_value(a[k], p);
+    const double xk = row_value(b, p, base) / row_value(a[k], p, base);
     if (pos < k) b = __builtin_fma(-a[k], xk, b);
-    if (lane == k) x = xk;
+    if (i == k) x = xk;
   }
   return x;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
+// The largest value over this lane's system.
+__device__ __forceinline__ double system_max(double v, int base) {
+#if P == 1
+  (void)base;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
+#else
+  double m = v;
+#pragma unroll
+  for (int r = 0; r < M; r++) m = fmax(m, __shfl(v, base + r));
+  return m;
+#endif
 }
 
 extern "C" __global__ void __launch_bounds__(64)
@@ -56,67 +79,92 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
            double* __restrict__ ys, double* __restrict__ k1, double* __restrict__ k2, double* __restrict__ F1,
            double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
            int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
-  const unsigned s = blockIdx.x;
   const int lane = threadIdx.x;
+  const int sys = lane / M, i = lane - sys * M, base = sys * M;
+  const bool row = sys < P;  // lanes past P M hold no row (with P = 1: the lanes past M)
+  const unsigned s = blockIdx.x * P + (unsigned)(row ? sys : 0);
+  const bool valid = row && s < n;
+  const unsigned sv = valid ? s : 0u;  // an in-bounds system for the loads of lanes that store nothing
+#if P == 1
   if (s >= n || status[s] != 0) return;  // uniform over the wave
-  const bool row = lane < M;
-  const int i = row ? lane : 0;
-  double hs = h[s];
+  bool active = row;
+#else
+  bool active = valid && status[sv] == 0;
+#endif
+  double hs = h[sv];
   if (phase == 1) {
-    const double remaining = t_end - t[s];
-    if (remaining <= 1e-12 * fabs(t_end)) {
-      if (lane == 0) status[s] = 1;
-      return;
+    const double remaining = t_end - t[sv];
+    if (active && remaining <= 1e-12 * fabs(t_end)) {
+      if (i == 0) status[s] = 1;
+      active = false;
     }
     hs = fmin(hs, remaining);  // the step ends at t_end at the latest
-    if (lane == 0) h[s] = hs;
+    if (active && i == 0) h[s] = hs;
   }
-  int* perm_s = perm_all + (size_t)s * M;
-  double* lu_s = lu + (size_t)s * M * M;
+#if P == 1
+  if (phase == 1 && !__builtin_amdgcn_readfirstlane((int)active)) return;  // the system is at t_end (lane 0 is a row)
+#endif
+  int* perm_s = perm_all + (size_t)sv * M;
+  double* lu_s = lu + (size_t)sv * M * M;
   double a[M];
-  int perm[M];  // wave-uniform
+  int perm[M];  // the same in every lane of a system (wave-uniform when P = 1)
   int pos = -1;
+  const int ir = row ? i : 0;
 
   if (phase == 1) {
     // W = I - h d J, factored with partial pivoting
 #pragma unroll
-    for (int j = 0; j < M; j++) a[j] = (row ? -hs * D_ROS * jac[(size_t)(M + i * M + j) * n + s] : 0.0) + (row && i == j ? 1.0 : 0.0);
+    for (int j = 0; j < M; j++) a[j] = (row ? -hs * D_ROS * jac[(size_t)(M + ir * M + j) * n + sv] : 0.0) + (row && i == j ? 1.0 : 0.0);
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      // pivot: the largest |a[k]| over the rows not yet chosen (ties to the lowest lane)
-      double key = row && pos < 0 ? fabs(a[k]) : -1.0;
+      // pivot: the largest |a[k]| over the rows not yet chosen (ties to the lowest row)
+      const double key = row && pos < 0 ? fabs(a[k]) : -1.0;
+#if P == 1
+      double best = key;
       int who = lane;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
-        const double ok = __shfl_xor(key, o);
+        const double ok = __shfl_xor(best, o);
         const int ow = __shfl_xor(who, o);
-        if (ok > key || (ok == key && ow < who)) {
-          key = ok;
+        if (ok > best || (ok == best && ow < who)) {
+          best = ok;
           who = ow;
         }
       }
       const int p = __builtin_amdgcn_readfirstlane(who);
+#else
+      double best = -2.0;
+      int p = 0;
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        const double kr = __shfl(key, base + r);
+        if (kr > best) {
+          best = kr;
+          p = r;
+        }
+      }
+#endif
       perm[k] = p;
-      if (lane == 0) perm_s[k] = p;
-      if (lane == p) pos = k;
-      // the pivot row's entries are read from lane p by every lane (outside the branch: the pivot lane's row is final)
-      const double pivot = lane_value(a[k], p);
+      if (active && i == 0) perm_s[k] = p;
+      if (row && i == p) pos = k;
+      // the pivot row's entries are read from its lane by every lane (outside the branch: the pivot row is final)
+      const double pivot = row_value(a[k], p, base);
       const bool eliminate = row && pos < 0;
       const double l = eliminate ? a[k] / pivot : 0.0;
       if (eliminate) a[k] = l;
 #pragma unroll
       for (int j = k + 1; j < M; j++) {
-        const double pj = lane_value(a[j], p);
+        const double pj = row_value(a[j], p, base);
         if (eliminate) a[j] = __builtin_fma(-l, pj, a[j]);
       }
     }
-    if (row) {
+    if (active) {
 #pragma unroll
       for (int j = 0; j < M; j++) lu_s[i * M + j] = a[j];
     }
-    const double f0 = row ? jac[(size_t)i * n + s] : 0.0;
-    const double x = solve(a, pos, perm, f0, lane);
-    if (row) {
+    const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
+    const double x = solve(a, pos, perm, f0, i, base);
+    if (active) {
       k1[(size_t)i * n + s] = x;
       ys[(size_t)i * n + s] = y[(size_t)i * n + s] + 0.5 * hs * x;
     }
@@ -125,44 +173,49 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
 
   // phases 2 and 3: the factors and each row's pivot step back from memory
 #pragma unroll
-  for (int j = 0; j < M; j++) a[j] = row ? lu_s[i * M + j] : 0.0;
+  for (int j = 0; j < M; j++) a[j] = row ? lu_s[ir * M + j] : 0.0;
 #pragma unroll
   for (int k = 0; k < M; k++) {
+#if P == 1
     perm[k] = __builtin_amdgcn_readfirstlane(perm_s[k]);
-    if (perm[k] == lane) pos = k;
+#else
+    perm[k] = perm_s[k];
+#endif
+    if (row && perm[k] == i) pos = k;
   }
-  const double x1 = row ? k1[(size_t)i * n + s] : 0.0;
+  const double x1 = row ? k1[(size_t)ir * n + sv] : 0.0;
   if (phase == 2) {
-    const double f1 = row ? F[(size_t)i * n + s] : 0.0;
-    const double x = solve(a, pos, perm, f1 - x1, lane) + x1;
-    if (row) {
+    const double f1 = row ? F[(size_t)ir * n + sv] : 0.0;
+    const double x = solve(a, pos, perm, f1 - x1, i, base) + x1;
+    if (active) {
       F1[(size_t)i * n + s] = f1;
       k2[(size_t)i * n + s] = x;
       ys[(size_t)i * n + s] = y[(size_t)i * n + s] + hs * x;
     }
     return;
   }
-  const double x2 = row ? k2[(size_t)i * n + s] : 0.0;
-  const double f0 = row ? jac[(size_t)i * n + s] : 0.0;
-  const double f1 = row ? F1[(size_t)i * n + s] : 0.0;
-  const double f2 = row ? F[(size_t)i * n + s] : 0.0;
-  const double x3 = solve(a, pos, perm, f2 - E32 * (x2 - f1) - 2.0 * (x1 - f0), lane);
-  const double y0 = row ? y[(size_t)i * n + s] : 0.0, y1 = row ? ys[(size_t)i * n + s] : 0.0;
+  const double x2 = row ? k2[(size_t)ir * n + sv] : 0.0;
+  const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
+  const double f1 = row ? F1[(size_t)ir * n + sv] : 0.0;
+  const double f2 = row ? F[(size_t)ir * n + sv] : 0.0;
+  const double x3 = solve(a, pos, perm, f2 - E32 * (x2 - f1) - 2.0 * (x1 - f0), i, base);
+  const double y0 = row ? y[(size_t)ir * n + sv] : 0.0, y1 = row ? ys[(size_t)ir * n + sv] : 0.0;
   const double err = hs * (x1 - 2.0 * x2 + x3) * (1.0 / 6.0);
   const double scale = (i == 0 ? atol0 : atol) + rtol * fmax(fabs(y0), fabs(y1));
   double ratio = row ? fabs(err) / scale : 0.0;
   if (!(ratio == ratio)) ratio = 1e300;  // a NaN (a singular W, say) rejects the step
-  ratio = wave_max(ratio);
+  ratio = system_max(ratio, base);
+  if (!active) return;
   const bool accept = ratio <= 1.0;
   const double factor = fmin(5.0, fmax(0.2, 0.9 * pow(fmax(ratio, 1e-12), -1.0 / 3.0)));
   double t_new = t[s];
   if (accept) {
-    if (row) y[(size_t)i * n + s] = y1;
+    y[(size_t)i * n + s] = y1;
     t_new += hs;
   }
   const double remaining = t_end - t_new;
   double h_new = hs * (accept ? factor : fmin(factor, 0.5));
-  if (lane == 0) {
+  if (i == 0) {
     t[s] = t_new;
     steps[s] += 1;
     if (remaining <= 1e-12 * fabs(t_end)) {
