@@ -1,7 +1,7 @@
 // Net production rates in FP64 for mechanisms too large for a straight-line kernel (hand-written; the host prepends the
 // mechanism sizes). A wave owns NS states and splits the reactions over G = 64/NS lane groups (lane = NS group +
-// state); NS is the largest power of two up to 8 whose per-state arrays fit about 10 KiB of LDS per wave, so that 16
-// waves share a CU (NS = 8 at 10 species, 4 at 53, 2 at 160):
+// state); NS is the largest power of two up to 8 whose per-state arrays fit 11 KiB of LDS per wave, so that about
+// 15 waves share a CU (NS = 8 at 10 species, 4 at 53, 2 at 160):
 //   1. per state, spread over the groups: concentrations C_k (reaction.rs:362-373 on a mass basis), the clamped
 //      exponentials E+_k = min(c+, e^g_k) and E-_k = min(c-, 1/e^g_k) of the Gibbs functions g_k = G°_k/RT of the
 //      state's NASA-7 piece (reaction.rs:236-268), and zeroed production rates, all in LDS ([species][state]);
@@ -13,13 +13,20 @@
 //      ω_k += ν_k q by LDS atomic adds (reaction.rs:312-327; the order of the additions differs from the reference's);
 //   3. the rates of the wave's states written out.
 // Tables (row = sorted, padded reaction; see rates_tiled_tables in tiled.cpp):
-//   rate[row] = (ln A, β, θ = Ea/R, -Σν); k0[row] = (ln A0 (Troe: ln(A0/A∞)), β0, θ0, A∞); troe[row] = (α, T3, T1, T2)
-//   (T1 >= 1e30: no α e^(-T/T1) term, α instead; T2 NaN: no e^(-T2/T) term);
-//   info[row] = (model | kinf_const << 4, first efficiency, efficiency count, 0), model 0 elementary, 1 irreversible,
-//   2 three-body, 3 Lindemann, 4 Troe, 5 padding; reactants/products[row] = up to 3 species (with multiplicity, -1
-//   none); net[row] = 6 16-bit entries 16 s + (ν + 4), two per word (low half first), 0xFFFF none;
-//   eff_species/eff_value = the (e_k - 1) lists; nasa[k] = the 2 x 7 coefficients of g_k (low piece first), split[k]
-//   the switch temperature (+inf: one piece).
+//   rate[row] = (ln A, β, θ = Ea/R, unused); k0[row] = (ln A0 (Troe: ln(A0/A∞)), β0, θ0, A∞); troe[row] = (α, T3, T1,
+//   T2) (T1 >= 1e30: no α e^(-T/T1) term, α instead; T2 NaN: no e^(-T2/T) term);
+//   info[row] = (model | kinf_const << 4, first efficiency, efficiency count, the round's largest list lengths:
+//   forward | reverse << 4 | targets << 8), model 0 elementary, 1 irreversible
+//   (the same path: its first reverse factor is the row of zeros), 2 three-body, 3 Lindemann, 4 Troe, 5 padding;
+//   eff_species/eff_value = the (e_k - 1) lists;
+//   ops[row] = 24 16-bit LDS byte offsets of rows of the per-state arrays (a lane adds 8 × its state): 4 forward
+//   factors (the reactants' C, with multiplicity), 12 reverse factors (E+ for each unit of a product-side ν, E- for
+//   each unit of a reactant-side ν, the products' C, and the row of (P0/RT)^(-Σν): 1, C0, 1/C0 or 1/C0²; an
+//   irreversible reaction's first reverse factor is the row of zeros), padded with the row of ones; 6 targets of
+//   ν_k q (the ω_k rows, padded with a row nobody reads), their ν + 8 in 4 bits each in the last word;
+//   nasa[k] = the 2 x 7 coefficients of g_k (low piece first), split[k] the switch temperature (+inf: one piece).
+// LDS rows ([row][state], NS doubles each): C_0..C_{K-1}, E+_0.., E-_0.., ω_0.., the spare ω row, ones, zeros,
+// C0, 1/C0, 1/C0² (K + 3 KA + 6 rows).
 // Required defines: K (species), KA (active species), ROUNDS (rounds of G reactions), NS (states per wave: 1, 2, 4, 8).
 
 #define G (64 / NS)
@@ -62,36 +69,39 @@ __device__ __forceinline__ double group_sum(double v) {
 
 // The table row of one reaction that every round reads.
 struct Row {
-  i32x4 info, re, pr, net;
+  i32x4 info, o0, o1, o2;
   f64x4 rate;
 };
 __device__ __forceinline__ Row load_row(int row, const i32x4* __restrict__ info, const f64x4* __restrict__ rate,
-                                        const i32x4* __restrict__ reactants, const i32x4* __restrict__ products,
-                                        const i32x4* __restrict__ net) {
+                                        const i32x4* __restrict__ ops) {
   Row r;
   r.info = info[row];
   r.rate = rate[row];
-  r.re = reactants[row];
-  r.pr = products[row];
-  r.net = net[row];
+  r.o0 = ops[3 * row];
+  r.o1 = ops[3 * row + 1];
+  r.o2 = ops[3 * row + 2];
   return r;
 }
-// Entry i (0..5) of a row's net changes: 16 s + (ν + 4), or 0xFFFF.
-__device__ __forceinline__ int net_entry(const i32x4& w, int i) { return (w[i >> 1] >> (16 * (i & 1))) & 0xFFFF; }
+// The double at LDS byte offset half h of word w (low half first), in this lane's state column.
+__device__ __forceinline__ double lds_at(const char* column, int w, int h) {
+  return *(const double*)(column + ((w >> (16 * h)) & 0xFFFF));
+}
 
+#define ROWS (K + 3 * KA + 6)
+#define ROW_SPARE (K + 3 * KA)
+#define ROW_ONE (ROW_SPARE + 1)
+#define ROW_ZERO (ROW_SPARE + 2)
+#define ROW_C0 (ROW_SPARE + 3)
 struct Lds {
-  double C[K][NS];
-  double Ep[KA][NS];
-  double En[KA][NS];
-  double W[KA][NS];
+  double v[ROWS][NS];
 };
 
 // at most 128 registers: four waves per SIMD, the 16 per CU that the LDS sizing allows
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n,
             const f64x4* __restrict__ rate, const f64x4* __restrict__ k0, const f64x4* __restrict__ troe,
-            const i32x4* __restrict__ info, const i32x4* __restrict__ reactants, const i32x4* __restrict__ products,
-            const i32x4* __restrict__ net, const int* __restrict__ eff_species, const double* __restrict__ eff_value,
+            const i32x4* __restrict__ info, const i32x4* __restrict__ ops, const int* __restrict__ eff_species,
+            const double* __restrict__ eff_value,
             const double* __restrict__ rcp_W, const double* __restrict__ nasa, const double* __restrict__ split,
             double clamp_pos, double clamp_neg) {
   __shared__ Lds L;
@@ -104,7 +114,7 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
   const unsigned s = group * NS + col;
   const bool valid = s < n;
   // the first round's rows, in flight during the per-state stage
-  Row cur = load_row(grp, info, rate, reactants, products, net);
+  Row cur = load_row(grp, info, rate, ops);
   // states past n compute on a harmless stand-in and write nothing
   const double T = valid ? in[0][s] : 1000.0, P = valid ? in[1][s] : 101325.0;
   const double rcpT = 1.0 / T, lnT = log(T), T2 = T * T;
@@ -118,36 +128,48 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
   double nonbulk = 0.0;
   for (int k = grp; k < K - 1; k += G) {
     const double c = density * fmax(0.0, (valid ? in[2 + k][s] : 1.0 / K) * rcp_W[k]);
-    L.C[k][col] = c;
+    L.v[k][col] = c;
     nonbulk += c;
   }
   nonbulk = group_sum(nonbulk);
-  if (grp == 0) L.C[K - 1][col] = C_tot - nonbulk;
+  if (grp == 0) L.v[K - 1][col] = C_tot - nonbulk;
   // Gibbs functions of the active species: g = a0 - a6 - a0 ln T - a1/2 T + ... + a5/T, as reaction.rs:78-81 arranges
   // the coefficients (the host stores them so arranged), one piece by T <= split
   for (int k = grp; k < KA; k += G) {
     const double* a = nasa + 14 * k + (T <= split[k] ? 0 : 7);
     const double g = a[0] + a[1] * lnT + a[2] * T + a[3] * T2 + a[4] * (T2 * T) + a[5] * (T2 * T2) + a[6] * rcpT;
     const double e = exp64(g);
-    L.Ep[k][col] = fmin(clamp_pos, e);
-    L.En[k][col] = fmin(clamp_neg, 1.0 / e);
-    L.W[k][col] = 0.0;
+    L.v[K + k][col] = fmin(clamp_pos, e);
+    L.v[K + KA + k][col] = fmin(clamp_neg, 1.0 / e);
+    L.v[K + 2 * KA + k][col] = 0.0;
+  }
+  if (grp == 0) {
+    const double rcpC0 = T * (1.0 / P0_R);
+    L.v[ROW_SPARE][col] = 0.0;
+    L.v[ROW_ONE][col] = 1.0;
+    L.v[ROW_ZERO][col] = 0.0;
+    L.v[ROW_C0][col] = P0_R * rcpT;
+    L.v[ROW_C0 + 1][col] = rcpC0;
+    L.v[ROW_C0 + 2][col] = rcpC0 * rcpC0;
   }
   __syncthreads();
-
-  const double C0 = P0_R * rcpT, rcpC0 = T * (1.0 / P0_R);
+  const char* column = (const char*)&L.v[0][col];
   // 2. rounds of G reactions
   for (int j = 0; j < ROUNDS; j++) {
     const int row = G * j + grp;
     const Row r = cur;
-    if (j + 1 < ROUNDS) cur = load_row(row + G, info, rate, reactants, products, net);
-    const int model = r.info[0] & 15;  // the round's model, or 5 on the padding rows that end a model's last round
+    if (j + 1 < ROUNDS) cur = load_row(row + G, info, rate, ops);
+    // the round's list lengths (the same in each of its rows) and model (5 on the padding rows that end a model's
+    // last round)
+    const int lengths = __builtin_amdgcn_readfirstlane(r.info[3]);
+    const int nf = lengths & 15, nr = (lengths >> 4) & 15, na = (lengths >> 8) & 15;
+    const int model = r.info[0] & 15;
     if (model == 5) continue;
     const f64x4 c = r.rate;
     double kf = fmin(FLT_MAX_D, exp64(c[0] + c[1] * lnT - c[2] * rcpT));
     if (model >= 2) {
       double M = C_tot;
-      for (int e = r.info[1]; e < r.info[1] + r.info[2]; e++) M = __builtin_fma(eff_value[e], L.C[eff_species[e]][col], M);
+      for (int e = r.info[1]; e < r.info[1] + r.info[2]; e++) M = __builtin_fma(eff_value[e], L.v[eff_species[e]][col], M);
       if (model == 2) {
         kf *= M;
       } else {
@@ -173,39 +195,29 @@ rates_tiled(const double* const* __restrict__ in, double* const* __restrict__ ou
         }
       }
     }
-    double q = kf;
+    // q = k_f Π C^ν' - k_f (1/K_c) Π C^ν'' from the row's lists of LDS rows, up to the round's lengths (a row's
+    // entries past its own lengths are the row of ones)
+    double f = kf, b = kf;
 #pragma unroll
-    for (int i = 0; i < 3; i++)
-      if (r.re[i] >= 0) q *= L.C[r.re[i]][col];
-    if (model != 1) {
-      double b = kf * (c[3] == 0.0 ? 1.0 : c[3] == 1.0 ? C0 : c[3] == -1.0 ? rcpC0 : rcpC0 * rcpC0);
+    for (int h = 0; h < 4; h++)
+      if (h < nf) f *= lds_at(column, r.o0[h >> 1], h & 1);
 #pragma unroll
-      for (int i = 0; i < 6; i++) {
-        const int e = net_entry(r.net, i);
-        if (e != 0xFFFF) {
-          const int sp = e >> 4, nu = (e & 15) - 4;
-          const double f = nu > 0 ? L.Ep[sp][col] : L.En[sp][col];
-          const int m = nu > 0 ? nu : -nu;  // 1 to 4
-          double fm = f;
-          if (m >= 2) fm *= f;
-          if (m >= 3) fm *= f;
-          if (m >= 4) fm *= f;
-          b *= fm;
-        }
+    for (int h = 0; h < 12; h++)
+      if (h < nr) b *= lds_at(column, h < 4 ? r.o0[2 + (h >> 1)] : r.o1[(h - 4) >> 1], h & 1);
+    const double q = f - b;
+    // ω_k += ν_k q for the row's targets (a row's targets past its own count are the spare row, with ν = 0)
+#pragma unroll
+    for (int h = 0; h < 6; h++) {
+      if (h < na) {
+        const int w = h < 4 ? r.o2[h >> 1] : r.o2[2];
+        const int off = (w >> (16 * (h & 1))) & 0xFFFF;
+        const double nu = (double)(((r.o2[3] >> (4 * h)) & 15) - 8);
+        atomicAdd((double*)(column + off), nu * q);
       }
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-        if (r.pr[i] >= 0) b *= L.C[r.pr[i]][col];
-      q -= b;
-    }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      const int e = net_entry(r.net, i);
-      if (e != 0xFFFF) atomicAdd(&L.W[e >> 4][col], (double)((e & 15) - 4) * q);
     }
   }
   __syncthreads();
   // 3. outputs
   if (valid)
-    for (int k = grp; k < KA; k += G) out[k][s] = L.W[k][col];
+    for (int k = grp; k < KA; k += G) out[k][s] = L.v[K + 2 * KA + k][col];
 }

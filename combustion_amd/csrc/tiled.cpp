@@ -5,6 +5,7 @@
 // transport_tiled (kernels/transport_tiled.hip) evaluates the Wilke sums as three K×K MFMA contractions and the
 // binary-diffusion polynomials as (pairs × D)·(D × states) MFMA tiles.
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -133,7 +134,6 @@ MixedTables mixed_tables(const Mechanism& m) {
     t.rate[4 * r] = std::log2(k.preexponential_factor);
     t.rate[4 * r + 1] = k.temperature_exponent;
     t.rate[4 * r + 2] = k.activation_temperature * kLog2e;
-    t.rate[4 * r + 3] = -x.sum_net;
     int kinf_const = 0;
     if (type >= 3) {
       // Troe factors A∞ out of k0 (reaction.rs:163-170)
@@ -210,7 +210,7 @@ std::string rates_mixed_text(const MixedTables& t) {
 struct TiledRatesTables {
   int K, KA, ROUNDS, NS;
   std::vector<double> rate, k0, troe, eff_value, rcp_W, nasa, split;
-  std::vector<int32_t> info, reactants, products, net, eff_species;
+  std::vector<int32_t> info, ops, eff_species;
 };
 
 TiledRatesTables rates_tiled_tables(const Mechanism& m) {
@@ -218,10 +218,12 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
   const Species& sp = m.species;
   t.K = (int)sp.len();
   t.KA = (int)m.active;
-  // states per wave: the largest power of two up to 8 whose LDS arrays (C, E+, E-, ω) fit 10 KiB, for 16 waves per CU
-  const size_t per_state = ((size_t)t.K + 3 * (size_t)t.KA) * 8;
+  // states per wave: the largest power of two up to 8 whose LDS rows (C, E+, E-, ω and six more) fit 11 KiB, for
+  // about 15 waves per CU
+  const size_t rows_lds = (size_t)t.K + 3 * (size_t)t.KA + 6;
+  const size_t per_state = rows_lds * 8;
   t.NS = 8;
-  while (t.NS > 1 && t.NS * per_state > 10240) t.NS /= 2;
+  while (t.NS > 1 && t.NS * per_state > 11264) t.NS /= 2;
   if (t.NS * per_state > 64 * 1024) throw Unsupported("mechanism too large for the tiled FP64 rates kernel's LDS");
   const int R = (int)m.reactions.size(), G = 64 / t.NS;
   auto model_code = [](const Reaction& x) {
@@ -235,9 +237,25 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
       default: throw Unsupported("the tiled FP64 rates kernel has no SRI falloff model (the straight-line forms do)");
     }
   };
+  // entries of a reaction's factor and target lists (see ops below): forward, reverse, targets
+  auto entries = [&](const Reaction& x) {
+    int nf = 0, nr = 1, na = 0;
+    for (int sp = 0; sp < t.K; sp++) nf += x.reactants[sp];
+    if (model_code(x) != 1) {
+      for (size_t sp = 0; sp < x.net.size(); sp++) nr += std::abs((int)x.net[sp]);
+      for (int sp = 0; sp < t.K; sp++) nr += x.products[sp];
+    }
+    for (size_t sp = 0; sp < x.net.size(); sp++) na += x.net[sp] != 0;
+    return std::array<int, 3>{nf, nr, na};
+  };
+  // rows sorted by model, and within a model by their list lengths, so that a round's rows have similar counts
   std::vector<int> order(R);
   for (int r = 0; r < R; r++) order[r] = r;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return model_code(m.reactions[a]) < model_code(m.reactions[b]); });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int ma = model_code(m.reactions[a]), mb = model_code(m.reactions[b]);
+    if (ma != mb) return ma < mb;
+    return entries(m.reactions[a]) < entries(m.reactions[b]);
+  });
   std::vector<int> rows;
   for (size_t i = 0; i < order.size();) {
     const int code = model_code(m.reactions[order[i]]);
@@ -251,10 +269,23 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
   t.k0.assign(4 * RP, 0.);
   t.troe.assign(4 * RP, 0.);
   t.info.assign(4 * RP, 0);
-  t.reactants.assign(4 * RP, -1);
-  t.products.assign(4 * RP, -1);
-  t.net.assign(4 * RP, -1);  // 0xFFFF halves: no entry
+  // LDS rows of the kernel, as byte offsets of the row (a lane adds 8 × its state column)
+  const int row_C = 0, row_Ep = t.K, row_En = t.K + t.KA, row_W = t.K + 2 * t.KA, row_spare = t.K + 3 * t.KA;
+  const int row_one = row_spare + 1, row_zero = row_spare + 2, row_C0 = row_spare + 3;
+  auto offset = [&](int lds_row) { return (uint32_t)(lds_row * t.NS * 8); };
+  t.ops.assign(12 * RP, 0);
+  auto put = [&](size_t r, int half, uint32_t value) {
+    uint32_t& word = reinterpret_cast<uint32_t&>(t.ops[12 * r + half / 2]);
+    const uint32_t shift = 16 * (half % 2);
+    word = (word & ~(0xFFFFu << shift)) | (value << shift);
+  };
+  auto fill_padding = [&](size_t r) {
+    for (int h = 0; h < 16; h++) put(r, h, offset(row_one));
+    for (int h = 16; h < 22; h++) put(r, h, offset(row_spare));
+    t.ops[12 * r + 11] = 0x888888;  // ν + 8 = 8 for the six targets
+  };
   for (size_t r = 0; r < RP; r++) {
+    fill_padding(r);
     if (rows[r] < 0) {
       t.info[4 * r] = 5;
       continue;
@@ -267,7 +298,6 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
     t.rate[4 * r] = std::log(k.preexponential_factor);
     t.rate[4 * r + 1] = k.temperature_exponent;
     t.rate[4 * r + 2] = k.activation_temperature;
-    t.rate[4 * r + 3] = -x.sum_net;
     int kinf_const = 0;
     if (code >= 3) {
       // Troe factors A∞ out of k0 (reaction.rs:163-170); Lindemann takes both limits whole
@@ -293,26 +323,39 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
         }
       t.info[4 * r + 2] = (int32_t)t.eff_species.size() - t.info[4 * r + 1];
     }
-    auto side = [&](const std::vector<uint8_t>& nu, std::vector<int32_t>& out) {
-      int n = 0;
-      for (int s = 0; s < t.K; s++)
-        for (int c = 0; c < nu[s]; c++) {
-          if (n == 3) throw Unsupported("more than 3 reactants or products in reaction " + x.equation);
-          out[4 * r + n++] = s;
-        }
-    };
-    side(x.reactants, t.reactants);
-    side(x.products, t.products);
+    // forward factors: the reactants' concentrations with multiplicity (halves 0-3)
+    int h = 0;
+    for (int sp = 0; sp < t.K; sp++)
+      for (int c = 0; c < x.reactants[sp]; c++) {
+        if (h == 4) throw Unsupported("more than 4 reactants in reaction " + x.equation);
+        put(r, h++, offset(row_C + sp));
+      }
+    // reverse factors (halves 4-15): E± per unit of net ν, the products' concentrations, (P0/RT)^(-Σν)
+    std::vector<uint32_t> rev;
+    if (code == 1) {
+      rev.push_back(offset(row_zero));
+    } else {
+      for (size_t sp = 0; sp < x.net.size(); sp++)
+        for (int c = 0; c < std::abs((int)x.net[sp]); c++) rev.push_back(offset((x.net[sp] > 0 ? row_Ep : row_En) + (int)sp));
+      for (int sp = 0; sp < t.K; sp++)
+        for (int c = 0; c < x.products[sp]; c++) rev.push_back(offset(row_C + sp));
+      const int power = -x.sum_net;  // 1, 0, -1 or -2 (checked above)
+      rev.push_back(offset(power == 1 ? row_C0 : power == 0 ? row_one : power == -1 ? row_C0 + 1 : row_C0 + 2));
+    }
+    if (rev.size() > 12) throw Unsupported("more than 12 reverse-rate factors in reaction " + x.equation);
+    for (size_t i = 0; i < rev.size(); i++) put(r, 4 + (int)i, rev[i]);
+    // targets of ν_k q (halves 16-21) and their ν + 8 (4 bits each, word 11)
     int n = 0;
-    for (size_t s = 0; s < x.net.size(); s++) {
-      const int nu = x.net[s];
+    uint32_t nus = 0x888888;
+    for (size_t sp = 0; sp < x.net.size(); sp++) {
+      const int nu = x.net[sp];
       if (!nu) continue;
-      if (n == 6 || std::abs(nu) > 4) throw Unsupported("too many or too large stoichiometric changes in reaction " + x.equation);
-      uint32_t& word = reinterpret_cast<uint32_t&>(t.net[4 * r + n / 2]);
-      const uint32_t entry = (uint32_t)s * 16 + (uint32_t)(nu + 4), shift = 16 * (n % 2);
-      word = (word & ~(0xFFFFu << shift)) | (entry << shift);
+      if (n == 6 || std::abs(nu) > 7) throw Unsupported("too many or too large stoichiometric changes in reaction " + x.equation);
+      put(r, 16 + n, offset(row_W + (int)sp));
+      nus = (nus & ~(0xFu << (4 * n))) | ((uint32_t)(nu + 8) << (4 * n));
       n++;
     }
+    t.ops[12 * r + 11] = (int32_t)nus;
   }
   if (t.eff_species.empty()) {  // keep the device arrays non-empty
     t.eff_species.push_back(0);
@@ -328,6 +371,19 @@ TiledRatesTables rates_tiled_tables(const Mechanism& m) {
     }
     t.split.push_back(std::isnan(nasa.temperature_split) ? INFINITY : nasa.temperature_split);
   }
+  // each round's largest list lengths, in every row of the round (info word 3: forward | reverse << 4 | targets << 8),
+  // so that the kernel skips the padding beyond them with wave-uniform branches
+  for (int round = 0; round < t.ROUNDS; round++) {
+    std::array<int, 3> most{0, 0, 0};
+    for (int g = 0; g < G; g++) {
+      const int row = rows[(size_t)round * G + g];
+      if (row < 0) continue;
+      const auto e = entries(m.reactions[row]);
+      for (int i = 0; i < 3; i++) most[i] = std::max(most[i], e[i]);
+    }
+    for (int g = 0; g < G; g++) t.info[4 * ((size_t)round * G + g) + 3] = most[0] | most[1] << 4 | most[2] << 8;
+  }
+  if (rows_lds * t.NS * 8 > 65535) throw Unsupported("mechanism too large for the tiled FP64 rates kernel's LDS offsets");
   return t;
 }
 
@@ -497,9 +553,7 @@ class RatesTiled {
     d_k0_ = upload(t_.k0);
     d_troe_ = upload(t_.troe);
     d_info_ = upload(t_.info);
-    d_reactants_ = upload(t_.reactants);
-    d_products_ = upload(t_.products);
-    d_net_ = upload(t_.net);
+    d_ops_ = upload(t_.ops);
     d_eff_species_ = upload(t_.eff_species);
     d_eff_value_ = upload(t_.eff_value);
     d_rcp_W_ = upload(t_.rcp_W);
@@ -507,15 +561,15 @@ class RatesTiled {
     d_split_ = upload(t_.split);
   }
   ~RatesTiled() {
-    for (void* ptr : {(void*)d_rate_, (void*)d_k0_, (void*)d_troe_, (void*)d_info_, (void*)d_reactants_, (void*)d_products_, (void*)d_net_,
-                      (void*)d_eff_species_, (void*)d_eff_value_, (void*)d_rcp_W_, (void*)d_nasa_, (void*)d_split_})
+    for (void* ptr : {(void*)d_rate_, (void*)d_k0_, (void*)d_troe_, (void*)d_info_, (void*)d_ops_, (void*)d_eff_species_,
+                      (void*)d_eff_value_, (void*)d_rcp_W_, (void*)d_nasa_, (void*)d_split_})
       if (ptr) (void)hipFree(ptr);
   }
   void launch(const double* const* in, double* const* out, size_t n, hipStream_t stream) {
     unsigned n32 = (unsigned)n;
     double pos = 6981463048192., neg = (double)cbrtf(3.40282347e38f);  // reaction.rs:244-256
-    void* args[] = {&in, &out, &n32, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_reactants_, &d_products_, &d_net_, &d_eff_species_,
-                    &d_eff_value_, &d_rcp_W_, &d_nasa_, &d_split_, &pos, &neg};
+    void* args[] = {&in, &out, &n32, &d_rate_, &d_k0_, &d_troe_, &d_info_, &d_ops_, &d_eff_species_, &d_eff_value_, &d_rcp_W_,
+                    &d_nasa_, &d_split_, &pos, &neg};
     const unsigned states_per_block = (unsigned)t_.NS;
     device::check(hipModuleLaunchKernel(kernel_.function, (unsigned)((n + states_per_block - 1) / states_per_block), 1, 1, 64, 1, 1, 0,
                                         stream, args, nullptr),
@@ -527,7 +581,7 @@ class RatesTiled {
   device::Kernel kernel_;
   double *d_rate_ = nullptr, *d_k0_ = nullptr, *d_troe_ = nullptr, *d_eff_value_ = nullptr, *d_rcp_W_ = nullptr, *d_nasa_ = nullptr,
          *d_split_ = nullptr;
-  int32_t *d_info_ = nullptr, *d_reactants_ = nullptr, *d_products_ = nullptr, *d_net_ = nullptr, *d_eff_species_ = nullptr;
+  int32_t *d_info_ = nullptr, *d_ops_ = nullptr, *d_eff_species_ = nullptr;
 };
 
 // rates_mixed with its tables in device memory; writes ω to out[0..Ka).
