@@ -134,6 +134,7 @@ MixedTables mixed_tables(const Mechanism& m) {
     t.rate[4 * r] = std::log2(k.preexponential_factor);
     t.rate[4 * r + 1] = k.temperature_exponent;
     t.rate[4 * r + 2] = k.activation_temperature * kLog2e;
+    t.rate[4 * r + 3] = -x.sum_net;
     int kinf_const = 0;
     if (type >= 3) {
       // Troe factors A∞ out of k0 (reaction.rs:163-170)
