@@ -354,6 +354,13 @@ def main():
     result["pcie_inclusive"] = {"states_per_s": N / host_s, "ms_per_step": host_s * 1e3,
                                 "note": "host numpy buffers through cb_hip_call (pageable memory, blocking); not the value"}
 
+    # the nekRK export (cuda.rs:296-337): the four emitted files compiled by hiprtc and run as one kernel over the same
+    # states (fits of length 5, the reference's export length), timed like the forms
+    exported = M.export_module(5)
+    wall_x, ms_x = time_module(exported, inputs, outputs, args.steps, args.warmup)
+    result["nekrk_export"] = {"states_per_s": N * args.steps / wall_x, "kernel_ms": ms_x, "vs_fp64_form": wall / wall_x,
+                              "note": "rates.c, conductivity.c, viscosity.c, diffusivity.c as emitted, one lane per state"}
+    del exported
     for form in ("mixed", "fast"):
         mod = prog.assemble(form=form)
         wall_f, ms_f = time_module(mod, inputs, outputs, args.steps, args.warmup)

@@ -98,15 +98,17 @@ __device__ __forceinline__ void half_tile(int h, int rt, int lane, int grp, int 
                                           const f32x4* __restrict__ troe, const i32x4* __restrict__ slots,
                                           const int* __restrict__ tile_info, const float* __restrict__ eff,
                                           double my_log2T, double my_rcpT, double my_log2C0, float my_T, float my_Ctot,
-                                          double* xf, double* xr) {
+                                          double (&xf)[4], double (&xr)[4]) {
   const int ht = 2 * rt + h;
   const int htype = tile_info[ht];  // the half's model, uniform over the wave
   if (htype == 5) {                 // padding rows only
+    #pragma unroll
     for (int i = 0; i < 4; i++) xf[i] = xr[i] = NO_TERM;
     return;
   }
   f64x4 kr[4];
   i32x4 sl[4];
+  #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = 16 * ht + 4 * grp + i;
     kr[i] = rate[r];
@@ -128,6 +130,7 @@ __device__ __forceinline__ void half_tile(int h, int rt, int lane, int grp, int 
     const float* A = eff + (size_t)ht * KE * 64;
     for (int kk = 0; kk < KE; kk++) Mv = __builtin_amdgcn_mfma_f32_16x16x4f32(A[kk * 64 + lane], L.C[kk * 4 + grp][col], Mv, 0, 0, 0);
   }
+  #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = 16 * ht + 4 * grp + i;
     const int info = sl[i][0];
@@ -160,6 +163,7 @@ __device__ __forceinline__ void half_tile(int h, int rt, int lane, int grp, int 
     }
     // log2(1/Kc) = Σ over the slots of G± + (-Σν) log2 C0
     double lK = kr[i][3] * my_log2C0;
+    #pragma unroll
     for (int w = 1; w < 4; w++) {
       const unsigned pair = (unsigned)sl[i][w];
       lK += *(const double*)(gcol + (pair & 0xFFFFu));
@@ -249,17 +253,25 @@ rates_mixed(const double* const* __restrict__ in, double* const* __restrict__ ou
   const char* gcol = (const char*)&L.G[0][0][col];
 
   f32x4 om[AB];
+  #pragma unroll
   for (int a = 0; a < AB; a++) om[a] = (f32x4){0.f, 0.f, 0.f, 0.f};
   int E = -100000;  // block exponent of this lane's state column
 
   for (int rt = 0; rt < RT; rt++) {
-    double xf[8], xr[8];
+    double xf0[4], xr0[4], xf1[4], xr1[4];  // the two halves' exponents (register arrays: constant indices only)
     half_tile(0, rt, lane, grp, col, L, gcol, Bh, Bm, Bl, A_fwd, A_rev, rate, k0, troe, slots, tile_info, eff, my_log2T, my_rcpT,
-              my_log2C0, my_T, my_Ctot, xf, xr);
+              my_log2C0, my_T, my_Ctot, xf0, xr0);
     half_tile(1, rt, lane, grp, col, L, gcol, Bh, Bm, Bl, A_fwd, A_rev, rate, k0, troe, slots, tile_info, eff, my_log2T, my_rcpT,
-              my_log2C0, my_T, my_Ctot, xf + 4, xr + 4);
+              my_log2C0, my_T, my_Ctot, xf1, xr1);
+    double xf[8], xr[8];
+    #pragma unroll
+    for (int i = 0; i < 4; i++) {
+      xf[i] = xf0[i], xr[i] = xr0[i];
+      xf[4 + i] = xf1[i], xr[4 + i] = xr1[i];
+    }
     // block exponent: the largest 2^x of the tile over the state's 4 lanes
     double xm = fmax(xf[0], xr[0]);
+    #pragma unroll
     for (int i = 1; i < 8; i++) xm = fmax(xm, fmax(xf[i], xr[i]));
     int e = xm > -1e299 ? (int)__builtin_ceil(xm) : -100000;
     e = max(e, __shfl_xor(e, 16));
@@ -267,12 +279,14 @@ rates_mixed(const double* const* __restrict__ in, double* const* __restrict__ ou
     if (e > E) {
       if (E > -100000) {
         const float f = __builtin_ldexpf(1.f, max(E - e, -126));
+        #pragma unroll
         for (int a = 0; a < AB; a++) om[a] *= f;
       }
       E = e;
     }
     const double Ed = (double)E;
     float q[8];
+    #pragma unroll
     for (int i = 0; i < 8; i++) {
       const float a = exp2f_hw((float)(xf[i] - Ed));
       const float d = (float)(xr[i] - xf[i]);  // FP64 difference of the two exponents
@@ -280,6 +294,7 @@ rates_mixed(const double* const* __restrict__ in, double* const* __restrict__ ou
     }
     // q = hi + mid + lo in BF16, the B operand of ν·q (element j of lane group g: reaction j < 4 ? 4g + j : 16 + 4g + j - 4)
     bf16x8 qh, qm, ql;
+    #pragma unroll
     for (int i = 0; i < 8; i += 2) {
       const f32x2 v = {q[i], q[i + 1]};
       const bf16x2 h = __builtin_convertvector(v, bf16x2);
@@ -291,6 +306,7 @@ rates_mixed(const double* const* __restrict__ in, double* const* __restrict__ ou
       qm[i] = m[0], qm[i + 1] = m[1];
       ql[i] = l[0], ql[i + 1] = l[1];
     }
+    #pragma unroll
     for (int a = 0; a < AB; a++) {
       const bf16x8 an = A_nu[(rt * AB + a) * 64 + lane];
       om[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(an, qh, om[a], 0, 0, 0);
@@ -299,7 +315,9 @@ rates_mixed(const double* const* __restrict__ in, double* const* __restrict__ ou
     }
   }
   if (!valid) return;
+  #pragma unroll
   for (int a = 0; a < AB; a++)
+    #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int k = 16 * a + 4 * grp + i;
       if (k < KA) out[k][s] = E > -100000 ? __builtin_ldexp((double)om[a][i], E) : 0.0;
