@@ -204,25 +204,36 @@ def large_mechanism_workload(args, dev):
     return out
 
 
-def integrator_workload(args, dev):
-    """the batched reactor integrator (cb_integrator_*, Rosenbrock ode23s with the analytic Jacobian)
-    on 2^16 constant-pressure reactors of the hand-written H2/O2 mechanism (tests/golden/handwritten.yaml: the twelve
-    initial states of tests/golden/reactor_handwritten.npz repeated with T0 scaled by U[0.95, 1.05]), integrated
-    through ignition to 1 ms at rtol 1e-8. Work per reactor step: two right-hand-side evaluations, one Jacobian
-    program (its own right-hand side included), the LU of the (K+1)² matrix and three solves."""
+def integrator_workload(args, dev, jacobian="automatic", mechanism="handwritten"):
+    """The batched reactor integrator (cb_integrator_*, Rosenbrock ode23s at rtol 1e-8; the Jacobian analytic or by
+    difference quotients, "automatic" choosing by the state size) on constant-pressure reactors:
+    - "handwritten": 2^16 reactors of the hand-written H2/O2 mechanism (K=6; tests/golden/handwritten.yaml, the
+      twelve initial states of tests/golden/reactor_handwritten.npz repeated with T0 scaled by U[0.95, 1.05]),
+      through ignition to 1 ms;
+    - "gri-53": 2^14 reactors of the 53-species mechanism (the four fuel/O2/N2 mixtures of
+      tests/golden/reactor_gri53.npz repeated with T0 scaled by U[0.98, 1.02]), to 0.1 ms.
+    Work per reactor step: two right-hand-side evaluations, the Jacobian (the analytic program with its own
+    right-hand side, or K+2 right-hand sides), the LU of the (K+1)² matrix and three solves."""
     import torch
-    from combustion_amd import Mechanism
+    from combustion_amd import Mechanism, synth
     from combustion_amd.native import Integrator
     golden = os.path.join(ROOT, "tests", "golden")
-    d = np.load(os.path.join(golden, "reactor_handwritten.npz"))
-    M = Mechanism(os.path.join(golden, "handwritten.yaml"))
-    n = 1 << 16
+    if mechanism == "handwritten":
+        d = np.load(os.path.join(golden, "reactor_handwritten.npz"))
+        M = Mechanism(os.path.join(golden, "handwritten.yaml"))
+        n, spread, t_end, warm = 1 << 16, 0.05, 1e-3, 1e-5
+        what = f"constant-pressure H2/O2 reactors (hand-written mechanism, K={M.K}), 0 -> 1 ms through ignition"
+    else:
+        d = np.load(os.path.join(golden, "reactor_gri53.npz"))
+        M = Mechanism(synth.mechanism_path(mechanism))
+        n, spread, t_end, warm = 1 << 14, 0.02, 1e-4, 1e-9
+        what = f"constant-pressure fuel/O2/N2 reactors ({mechanism} mechanism, K={M.K}), 0 -> 0.1 ms"
     reps = -(-n // len(d["T0"]))
     rng = np.random.default_rng(16)
-    T0 = np.tile(d["T0"], reps)[:n] * rng.uniform(0.95, 1.05, n)
+    T0 = np.tile(d["T0"], reps)[:n] * rng.uniform(1 - spread, 1 + spread, n)
     Y0 = np.tile(d["Y0"], (1, reps))[:, :n]
     P = np.tile(d["P"], reps)[:n]
-    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12)
+    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, jacobian=jacobian)
 
     def run(t_end):
         y = torch.from_numpy(np.vstack([T0, Y0])).to(dev).contiguous()
@@ -237,19 +248,23 @@ def integrator_workload(args, dev):
         torch.cuda.synchronize()
         return time.perf_counter() - t0, taken, status, steps, y
 
-    run(1e-5)  # warm-up: module loads
-    wall, taken, status, steps, y = run(1e-3)
+    run(warm)  # warm-up: module loads
+    wall, taken, status, steps, y = run(t_end)
     m = M.K + 1
     rhs = M.reactor()
-    jac = rhs.jacobian([0] + [2 + k for k in range(M.K)])
-    flop_step = 2 * rhs.op_counts()["weighted"] + jac.op_counts()["weighted"] + (2 * m ** 3) // 3 + 3 * 2 * m * m
+    used = Integrator.resolved_jacobian(M, jacobian)
+    if used == "differences":
+        jac_flop = (m + 1) * rhs.op_counts()["weighted"] + 3 * m * m
+    else:
+        jac_flop = rhs.jacobian([0] + [2 + k for k in range(M.K)]).op_counts()["weighted"]
+    flop_step = 2 * rhs.op_counts()["weighted"] + jac_flop + (2 * m ** 3) // 3 + 3 * 2 * m * m
     reactor_steps = int(steps.sum().item())
-    return {"workload": f"{n} constant-pressure H2/O2 reactors (hand-written mechanism, K={M.K}), 0 -> 1 ms through "
-                        f"ignition, rtol 1e-8", "reactors": n, "batch_steps": taken, "wall_ms": wall * 1e3,
+    rise = (y[0] - torch.from_numpy(T0).to(dev)).cpu().numpy()
+    return {"workload": f"{n} {what}, rtol 1e-8", "jacobian": used, "reactors": n, "batch_steps": taken, "wall_ms": wall * 1e3,
             "reactor_steps": reactor_steps, "reactor_steps_per_s": reactor_steps / wall,
             "steps_per_reactor": {"min": int(steps.min().item()), "max": int(steps.max().item())},
             "reached_t_end": float((status == 1).float().mean().item()),
-            "ignited": float((y[0] > torch.from_numpy(T0).to(dev) + 800).float().mean().item()),
+            "temperature_rise_K": {"min": float(rise.min()), "max": float(rise.max())},
             "flop_per_reactor_step": flop_step,
             "achieved_tflops": flop_step * reactor_steps / wall / 1e12}
 
@@ -268,6 +283,10 @@ def main():
     ap.add_argument("--mechanism", default=MECHANISM, help="for --only: gri-53 (default) or large-160")
     ap.add_argument("--no-large", action="store_true", help="leave out the 160-species workload")
     ap.add_argument("--no-integrator", action="store_true", help="leave out the batched reactor integrator")
+    ap.add_argument("--integrator-jacobian", default="automatic", choices=["automatic", "analytic", "differences"],
+                    help="for --only integrator")
+    ap.add_argument("--integrator-mechanism", default="handwritten", choices=["handwritten", "gri-53"],
+                    help="for --only integrator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every state is independent, so N GPUs would each run "
@@ -290,7 +309,8 @@ def main():
     prog = M.chemistry()
     outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
     if args.only == "integrator":
-        print(json.dumps({"only": "integrator", **integrator_workload(args, dev)}))
+        print(json.dumps({"only": "integrator", **integrator_workload(args, dev, args.integrator_jacobian,
+                                                                     args.integrator_mechanism)}))
         return
     if args.only == "surrogate":
         from combustion_amd import reactor
@@ -378,6 +398,10 @@ def main():
     if not args.no_integrator:
         log("batched reactor integrator")
         result["integrator"] = integrator_workload(args, dev)
+        log("batched reactor integrator, difference quotients")
+        result["integrator_differences"] = integrator_workload(args, dev, "differences")
+        log("batched reactor integrator, 53 species")
+        result["integrator_gri53"] = integrator_workload(args, dev, mechanism="gri-53")
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(prog, MECHANISM)
     print(json.dumps(result))

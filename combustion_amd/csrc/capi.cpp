@@ -544,13 +544,13 @@ int cb_cpu_call(cb_cpu_module* mod, const double* const* inputs, int32_t n_input
 
 void cb_cpu_module_free(cb_cpu_module* mod) { delete mod; }
 
-int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, cb_integrator** out) {
+int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, int32_t flags, cb_integrator** out) {
   return guard([&] {
     require(m, "null mechanism");
-    if (!out) return compile_integrator(m->m);
+    if (!out) return compile_integrator(m->m, flags);
     auto* it = new cb_integrator;
     try {
-      it->integrator = std::make_unique<Integrator>(m->m, rtol, atol_T, atol_Y);
+      it->integrator = std::make_unique<Integrator>(m->m, rtol, atol_T, atol_Y, flags);
     } catch (...) {
       delete it;
       throw;

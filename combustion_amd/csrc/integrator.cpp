@@ -1,4 +1,5 @@
-// Batched constant-pressure reactors on the GPU: per step, the analytic Jacobian program (f and J in one kernel), the
+// Batched constant-pressure reactors on the GPU: per step, f and J (the analytic Jacobian program in one kernel, or
+// difference quotients: a replicate kernel, the right-hand side over n (K+2) states and a difference kernel), the
 // three Rosenbrock phases of kernels/rosenbrock.hip and two right-hand-side evaluations, all on the caller's stream;
 // the batch is polled for reactors still running every few steps.
 #include "integrator.hpp"
@@ -21,42 +22,54 @@ constexpr int kPollEvery = 8;
 // Rosenbrock kernels' source, for a mechanism.
 struct IntegratorPrograms {
   int m;
+  bool differences;
   ast::Function rhs_function, jacobian_function;
   ast::Linear rhs_linear, jacobian_linear;
   std::string step_source;
-  explicit IntegratorPrograms(const Mechanism& mech) : m((int)mech.species.len() + 1) {
+  IntegratorPrograms(const Mechanism& mech, int flags) : m((int)mech.species.len() + 1) {
     if (m > 64) throw Unsupported("the batched integrator holds a reactor's state in one wave: at most 63 species");
+    const int jac = flags & kJacobianMask;
+    if ((flags & ~(kJacobianMask | kOneSystemPerWave)) || jac > kJacobianDifferences) throw ArgumentError("unknown integrator flags");
+    differences = jac == kJacobianDifferences || (jac == kJacobianAutomatic && m > kAnalyticMaxState);
     rhs_function = reactor(mech, false);
     rhs_linear = ast::linearize(rhs_function);
-    std::vector<int32_t> wrt = {0}, rows;
-    for (int k = 0; k + 1 < m; k++) wrt.push_back(2 + k);
-    for (int o = 0; o < m; o++) rows.push_back(o);
-    jacobian_linear = ast::jacobian(rhs_linear, wrt, rows);
-    jacobian_function.n_input = jacobian_linear.n_input;
-    jacobian_function.output.assign(jacobian_linear.output.size(), -1);
+    if (!differences) {
+      std::vector<int32_t> wrt = {0}, rows;
+      for (int k = 0; k + 1 < m; k++) wrt.push_back(2 + k);
+      for (int o = 0; o < m; o++) rows.push_back(o);
+      jacobian_linear = ast::jacobian(rhs_linear, wrt, rows);
+      jacobian_function.n_input = jacobian_linear.n_input;
+      jacobian_function.output.assign(jacobian_linear.output.size(), -1);
+    }
     std::ostringstream src;
-    src << "#define M " << m << "\n" << rosenbrock_source;
+    src << "#define M " << m << "\n";
+    if (flags & kOneSystemPerWave) src << "#define P 1\n";
+    src << rosenbrock_source;
     step_source = src.str();
   }
 };
 
-void compile_integrator(std::shared_ptr<const Mechanism> m) {
-  IntegratorPrograms p(*m);
+void compile_integrator(std::shared_ptr<const Mechanism> m, int flags) {
+  IntegratorPrograms p(*m, flags);
   compile_form(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, 1, 0);
-  compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, 0);
+  if (!p.differences) compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, 0);
   device::compile(p.step_source, "rosenbrock", hip_options(true));
 }
 
-Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double atol_T, double atol_Y)
+Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double atol_T, double atol_Y, int flags)
     : mech_(std::move(m)), m_((int)mech_->species.len() + 1), rtol_(rtol), atol_T_(atol_T), atol_Y_(atol_Y) {
   if (!(rtol > 0) || !(atol_T > 0) || !(atol_Y > 0)) throw ArgumentError("tolerances must be positive");
-  IntegratorPrograms p(*mech_);
+  IntegratorPrograms p(*mech_, flags);
+  differences_ = p.differences;
+  systems_per_wave_ = (flags & kOneSystemPerWave) ? 1u : 64u / (unsigned)m_;
   rhs_function_ = std::move(p.rhs_function);
   rhs_linear_ = std::move(p.rhs_linear);
-  jacobian_function_ = std::move(p.jacobian_function);
-  jacobian_linear_ = std::move(p.jacobian_linear);
   rhs_ = assemble_form(ProgramView{mech_, &rhs_function_, &rhs_linear_, ProgramKind::Other, nullptr}, 1, 0);
-  jacobian_ = assemble_form(ProgramView{mech_, &jacobian_function_, &jacobian_linear_, ProgramKind::Other, nullptr}, 1, 0);
+  if (!differences_) {
+    jacobian_function_ = std::move(p.jacobian_function);
+    jacobian_linear_ = std::move(p.jacobian_linear);
+    jacobian_ = assemble_form(ProgramView{mech_, &jacobian_function_, &jacobian_linear_, ProgramKind::Other, nullptr}, 1, 0);
+  }
   step_.load(device::compile(p.step_source, "rosenbrock", hip_options(true)), "rosenbrock");
 }
 
@@ -110,6 +123,17 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   tab_ys_ = table(inputs(ys_));
   tab_jac_ = table(rows(jac_, m + m * m));
   tab_F_ = table(rows(F_, m));
+  if (differences_) {
+    // M + 1 copies of every state: K + 2 input rows and M output rows of n (M + 1) doubles
+    const size_t nr = n * (m + 1);
+    double* rep_in = (double*)alloc((m + 1) * nr * sizeof(double));
+    double* rep_out = (double*)alloc(m * nr * sizeof(double));
+    std::vector<const void*> in_rows, out_rows;
+    for (size_t i = 0; i < m + 1; i++) in_rows.push_back(rep_in + i * nr);
+    for (size_t i = 0; i < m; i++) out_rows.push_back(rep_out + i * nr);
+    tab_rep_in_ = table(in_rows);
+    tab_rep_out_ = table(out_rows);
+  }
   n_ = n;
   last_y_ = y;
   last_P_ = P;
@@ -120,15 +144,28 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   if (n == 0) return 0;
   if (n >= (size_t(1) << 31)) throw ArgumentError("at most 2^31 - 1 reactors");
   prepare(y, P, n);
-  hipFunction_t running = nullptr, clear = nullptr;
+  hipFunction_t running = nullptr, clear = nullptr, replicate = nullptr, difference = nullptr;
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
+  device::check(hipModuleGetFunction(&replicate, step_.module, "rosenbrock_replicate"), "hipModuleGetFunction");
+  device::check(hipModuleGetFunction(&difference, step_.module, "rosenbrock_difference"), "hipModuleGetFunction");
   unsigned n32 = (unsigned)n;
-  const unsigned systems_per_wave = 64u / (unsigned)m_;  // kernels/rosenbrock.hip packs ⌊64/M⌋ systems per wave
+  const unsigned systems_per_wave = systems_per_wave_;
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
+  const size_t copies = n * (size_t)(m_ + 1), columns = n * (size_t)m_;
   long taken = 0;
   for (int step = 0; step < max_steps; step++) {
-    jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
+    if (differences_) {
+      void* rargs[] = {&n32, &y, &P, &tab_rep_in_, &rtol, &atol0, &atol};
+      device::check(hipModuleLaunchKernel(replicate, (unsigned)((copies + 255) / 256), 1, 1, 256, 1, 1, 0, stream, rargs, nullptr),
+                    "launch rosenbrock_replicate");
+      rhs_->launch_only((const double* const*)tab_rep_in_, (double* const*)tab_rep_out_, copies, stream, nullptr);
+      void* dargs[] = {&n32, &tab_rep_out_, &jac_, &y, &rtol, &atol0, &atol};
+      device::check(hipModuleLaunchKernel(difference, (unsigned)((columns + 255) / 256), 1, 1, 256, 1, 1, 0, stream, dargs, nullptr),
+                    "launch rosenbrock_difference");
+    } else {
+      jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
+    }
     for (int phase = 1; phase <= 3; phase++) {
       void* args[] = {&phase, &n32, &jac_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
       device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,

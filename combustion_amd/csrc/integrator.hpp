@@ -1,6 +1,6 @@
 // Batched stiff integration of constant-pressure reactors on the GPU (the role of cvode/lib.rs:17-47 and the
-// integration loop of plot.rs:80-118): the reactor right-hand side and its analytic Jacobian as FP64 straight-line
-// kernels, and the Rosenbrock stages in kernels/rosenbrock.hip.
+// integration loop of plot.rs:80-118): the reactor right-hand side and its Jacobian (the analytic program, or
+// difference quotients) as FP64 straight-line kernels, and the Rosenbrock stages in kernels/rosenbrock.hip.
 #pragma once
 #include <hip/hip_runtime_api.h>
 
@@ -14,14 +14,22 @@
 
 namespace combustion {
 
+// Options (cb_integrator_create's flags): the Jacobian, and the Rosenbrock kernel's packing.
+constexpr int kJacobianAutomatic = 0;    // the analytic program up to kAnalyticMaxState rows, difference quotients above
+constexpr int kJacobianAnalytic = 1;     // the forward-mode program of cb_build_jacobian
+constexpr int kJacobianDifferences = 2;  // difference quotients around one right-hand-side launch of n (K+2) states
+constexpr int kJacobianMask = 0xF;
+constexpr int kOneSystemPerWave = 0x10;  // one reactor per Rosenbrock wave (readlane broadcasts) whatever its size
+constexpr int kAnalyticMaxState = 8;  // beyond, the analytic program spills (3.2 KB per lane at 11 rows) and loses to differences
+
 // Fills the code-object cache with the integrator's kernels for a mechanism (no GPU needed).
-void compile_integrator(std::shared_ptr<const Mechanism> m);
+void compile_integrator(std::shared_ptr<const Mechanism> m, int flags);
 
 class Integrator {
  public:
   // Tolerances as CVODE's scalar pair (cvode/lib.rs:30): rtol relative; atol_T for the temperature and atol_Y for
   // the mass fractions.
-  Integrator(std::shared_ptr<const Mechanism> m, double rtol, double atol_T, double atol_Y);
+  Integrator(std::shared_ptr<const Mechanism> m, double rtol, double atol_T, double atol_Y, int flags);
   ~Integrator();
   int state_size() const { return m_; }
   // Advances n reactors to t_end in place, all arrays device memory: y [K+1][n] (T, then Y_0..Y_{K-1}, SoA), P [n],
@@ -35,6 +43,8 @@ class Integrator {
   std::shared_ptr<const Mechanism> mech_;
   int m_;
   double rtol_, atol_T_, atol_Y_;
+  bool differences_;
+  unsigned systems_per_wave_;
   ast::Function rhs_function_, jacobian_function_;
   ast::Linear rhs_linear_, jacobian_linear_;
   std::unique_ptr<Form> rhs_, jacobian_;
@@ -45,6 +55,7 @@ class Integrator {
   int* perm_ = nullptr;
   unsigned* running_ = nullptr;
   void **tab_y_ = nullptr, **tab_ys_ = nullptr, **tab_jac_ = nullptr, **tab_F_ = nullptr;
+  void **tab_rep_in_ = nullptr, **tab_rep_out_ = nullptr;  // difference quotients: the n (K+2) states and their f
   const double* last_y_ = nullptr;
   const double* last_P_ = nullptr;
   void release();

@@ -14,12 +14,16 @@
 // memory for phases 2 and 3. Systems whose status is not 0 (running) are left alone.
 // Layouts: vectors are SoA, component i of system s at [i * n + s]; jac holds f(y) in rows 0..M-1 and J row-major in
 // rows M..M+M²-1 (the Jacobian program's outputs); lu[s][i][j]; perm[s][k] = the row chosen at step k.
-// Required define: M. Launch: ⌈n / P⌉ blocks of 64 threads.
+// Required define: M; P (systems per wave) defaults to ⌊64/M⌋. Launch: ⌈n / P⌉ blocks of 64 threads.
+// With difference quotients instead of the Jacobian program, rosenbrock_replicate and rosenbrock_difference build
+// the same jac rows around one right-hand-side evaluation of n (M + 1) states.
 
 #define D_ROS 0.29289321881345248   // 1 / (2 + √2)
 #define E32 7.4142135623730950      // 6 + √2
 #define H_MIN 1e-30
-#define P (64 / M)                  // systems per wave
+#ifndef P
+#define P (64 / M)  // systems per wave
+#endif
 
 // The value a lane holds, for a wave-uniform lane index (two v_readlane_b32).
 __device__ __forceinline__ double lane_value(double v, int lane) {
@@ -225,6 +229,44 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
       if (h_new < H_MIN) status[s] = 2;  // the step size collapsed
       h[s] = h_new;
     }
+  }
+}
+
+// Difference quotients (the dense Jacobian of the reference's CVODE path, cvode/lib.rs:17-47): component j of a
+// system's state is increased by inc_j = √ε max(|y_j|, atol_j / rtol).
+__device__ __forceinline__ double increment(double yj, int j, double rtol, double atol0, double atol) {
+  return 1.4901161193847656e-08 * fmax(fabs(yj), (j == 0 ? atol0 : atol) / rtol);
+}
+
+// Copies c = 0..M of every system's state as inputs [T, P, Y_0..Y_{M-2}] of the right-hand-side program over n (M + 1)
+// states (copy c at [c n, (c + 1) n)): copy 0 unchanged, copy j + 1 with component j increased by inc_j.
+extern "C" __global__ void __launch_bounds__(256)
+rosenbrock_replicate(unsigned n, const double* __restrict__ y, const double* __restrict__ P_in, double* const* __restrict__ rep,
+                     double rtol, double atol0, double atol) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * (M + 1)) return;
+  const unsigned c = (unsigned)(idx / n), s = (unsigned)(idx - (size_t)c * n);
+  for (int j = 0; j < M; j++) {
+    double v = y[(size_t)j * n + s];
+    if ((int)c == j + 1) v += increment(v, j, rtol, atol0, atol);
+    rep[j == 0 ? 0 : j + 1][idx] = v;
+  }
+  rep[1][idx] = P_in[s];
+}
+
+// jac rows 0..M-1 = f(y) (copy 0), rows M + i M + j = (f_i(copy j + 1) - f_i(y)) / inc_j; one thread per (j, system).
+extern "C" __global__ void __launch_bounds__(256)
+rosenbrock_difference(unsigned n, const double* const* __restrict__ F, double* __restrict__ jac, const double* __restrict__ y,
+                      double rtol, double atol0, double atol) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * M) return;
+  const int j = (int)(idx / n);
+  const unsigned s = (unsigned)(idx - (size_t)j * n);
+  const double rinc = 1.0 / increment(y[(size_t)j * n + s], j, rtol, atol0, atol);
+  for (int i = 0; i < M; i++) {
+    const double f0 = F[i][s];
+    if (j == 0) jac[(size_t)i * n + s] = f0;
+    jac[(size_t)(M + i * M + j) * n + s] = (F[i][(size_t)(j + 1) * n + s] - f0) * rinc;
   }
 }
 

@@ -92,7 +92,7 @@ SIGNATURES = {
     "cb_hip_call": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
     "cb_hip_call_static": (C.c_int, [_P, _DPP, C.c_int32, C.c_size_t, _DPP, C.c_int32, _P]),
     "cb_module_free": (None, [_P]),
-    "cb_integrator_create": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
+    "cb_integrator_create": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int32, C.POINTER(_P)]),
     "cb_integrator_advance": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int32, _P, _I64P]),
     "cb_integrator_free": (None, [_P]),
     "cb_cpu_assemble": (C.c_int, [_P, C.POINTER(_P)]),
@@ -413,19 +413,35 @@ class Module:
 
 
 class Integrator:
-    """cb_integrator_*: batched constant-pressure reactors on the GPU (Rosenbrock ode23s with the analytic Jacobian).
+    """cb_integrator_*: batched constant-pressure reactors on the GPU (Rosenbrock ode23s; the Jacobian analytic or by
+    difference quotients).
     advance(y, P, t, h, status, steps, t_end) on contiguous torch device tensors: y float64 [K+1][n] (T, Y_k), P, t, h
     float64 [n], status int32 [n], steps int32 [n]; returns the batch's step count."""
 
-    def __init__(self, mechanism, rtol=1e-6, atol_T=1e-6, atol_Y=1e-12):
+    JACOBIAN = {"automatic": 0, "analytic": 1, "differences": 2}
+    ANALYTIC_MAX_STATE = 8  # "automatic" takes the analytic program up to this state size (kAnalyticMaxState)
+
+    @classmethod
+    def resolved_jacobian(cls, mechanism, jacobian="automatic"):
+        """The Jacobian an integrator of this mechanism uses: "analytic" or "differences"."""
+        if jacobian != "automatic":
+            return jacobian
+        return "analytic" if mechanism.K + 1 <= cls.ANALYTIC_MAX_STATE else "differences"
+
+    @classmethod
+    def flags(cls, jacobian="automatic", one_per_wave=False):
+        """cb_integrator_flags from a Jacobian name and the Rosenbrock packing."""
+        return cls.JACOBIAN[jacobian] | (0x10 if one_per_wave else 0)
+
+    def __init__(self, mechanism, rtol=1e-6, atol_T=1e-6, atol_Y=1e-12, jacobian="automatic", one_per_wave=False):
         h = _P()
-        _check(lib().cb_integrator_create(mechanism._h, rtol, atol_T, atol_Y, C.byref(h)))
+        _check(lib().cb_integrator_create(mechanism._h, rtol, atol_T, atol_Y, self.flags(jacobian, one_per_wave), C.byref(h)))
         self._h = h
         self.mechanism = mechanism
 
-    @staticmethod
-    def compile(mechanism):
-        _check(lib().cb_integrator_create(mechanism._h, 1e-6, 1e-6, 1e-12, None))
+    @classmethod
+    def compile(cls, mechanism, jacobian="automatic", one_per_wave=False):
+        _check(lib().cb_integrator_create(mechanism._h, 1e-6, 1e-6, 1e-12, cls.flags(jacobian, one_per_wave), None))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -183,14 +183,23 @@ void cb_cpu_module_free(cb_cpu_module* mod);
 /* Batched stiff integration of constant-pressure reactors on the GPU, the counterpart of the reference's CVODE
  * wrapper (cvode/lib.rs:17-47: CVodeCreate(CV_BDF), CVodeSStolerances(rtol, atol), dense linear solver, one step at
  * a time) and of the reactor loop of plot.rs:80-118, for many reactors at once: the L-stable (2,3) Rosenbrock pair
- * of ode23s (Shampine & Reichelt 1997) with the analytic Jacobian of cb_build_reactor's program (cb_build_jacobian)
- * and one wave per reactor for the LU factorisation and the solves (state size K+1 <= 64). Tolerances: rtol, and
+ * of ode23s (Shampine & Reichelt 1997) with the Jacobian of cb_build_reactor's program (analytic, cb_build_jacobian,
+ * or difference quotients: flags) and floor(64/(K+1)) reactors per wave for the LU factorisation and the solves
+ * (state size K+1 <= 64). Tolerances: rtol, and
  * atol_T (temperature) / atol_Y (mass fractions). With out == NULL the kernels are only compiled into the cache.
  * cb_integrator_advance moves n reactors toward t_end, in place, every array in device memory: y [K+1][n] (SoA rows
  * T, Y_0..Y_{K-1}), P [n] (constant), t [n] and h [n] (each reactor's time and next step size), status [n] (0
  * running; set to 1 at t_end, 2 when the step size collapsed; others are left alone), steps [n] (attempted steps).
  * It stops after max_steps steps of the batch or when no reactor runs, and blocks until then. */
-int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, cb_integrator** out);
+enum cb_integrator_flags {
+  CB_JACOBIAN_AUTOMATIC = 0,   /* the analytic program up to 8 state rows (K <= 7), difference quotients above */
+  CB_JACOBIAN_ANALYTIC = 1,    /* cb_build_jacobian's forward-mode program, f and J in one straight-line kernel */
+  CB_JACOBIAN_DIFFERENCES = 2, /* difference quotients, as CVODE's dense solver forms them: every state copied K+2
+                                  times (one copy per perturbed component, inc_j = sqrt(eps) max(|y_j|, atol_j/rtol)),
+                                  one right-hand-side launch over the copies, one difference kernel */
+  CB_INTEGRATOR_ONE_PER_WAVE = 0x10 /* one reactor per Rosenbrock wave whatever its size (otherwise floor(64/(K+1))) */
+};
+int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, int32_t flags, cb_integrator** out);
 int cb_integrator_advance(cb_integrator* it, size_t n_reactors, double* y, const double* P, double* t, double* h, int32_t* status,
                           uint32_t* steps, double t_end, int32_t max_steps, void* stream, int64_t* steps_taken);
 void cb_integrator_free(cb_integrator* it);

@@ -1,10 +1,11 @@
-"""The batched reactor integrator on the GPU (cb_integrator_*: Rosenbrock ode23s with the analytic Jacobian, one wave
-per reactor for the LU and the solves), the counterpart of the reference's CVODE wrapper (cvode/lib.rs:17-47) and
-reactor loop (plot.rs:80-118): twelve constant-pressure H2/O2 reactors of the hand-written mechanism integrated
-through ignition, against the committed scipy BDF trajectories (tests/golden/make_reactor_golden.py, rtol 1e-10 on
-the FP64 oracle's right-hand side) at 1, 3 and 10 ms. Bars: with the integrator's tolerances at rtol 1e-8, atol 1e-6 K
-and 1e-12, temperatures within 2e-6 relative and mass fractions within 1e-6 absolute of the reference trajectory
-(measured: 1.6e-7 and 8.5e-8 at 1 ms, mid-ignition for some reactors, where timing differences weigh most)."""
+"""The batched reactor integrator on the GPU (cb_integrator_*: Rosenbrock ode23s, ⌊64/(K+1)⌋ reactors per wave for
+the LU and the solves), the counterpart of the reference's CVODE wrapper (cvode/lib.rs:17-47) and reactor loop
+(plot.rs:80-118): twelve constant-pressure H2/O2 reactors of the hand-written mechanism integrated through ignition,
+against the committed scipy BDF trajectories (tests/golden/make_reactor_golden.py, rtol 1e-10 on the FP64 oracle's
+right-hand side) at 1, 3 and 10 ms, with the analytic Jacobian and with difference quotients, several reactors per
+wave and one. Bars: with the integrator's tolerances at rtol 1e-8, atol 1e-6 K and 1e-12, temperatures within 2e-6
+relative and mass fractions within 1e-6 absolute of the reference trajectory (measured with the analytic Jacobian:
+1.6e-7 and 8.5e-8 at 1 ms, mid-ignition for some reactors, where timing differences weigh most)."""
 import os
 
 import numpy as np
@@ -17,11 +18,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
-@pytest.mark.gpu
-def test_reactors_through_ignition_match_scipy_bdf(gpu):
+def against_fixture(gpu, d, M, integ, dT_max, dY_max):
+    """Advances the fixture's reactors to each of its output times, checking every reactor's state there."""
     import torch
-    d = np.load(os.path.join(GOLDEN, "reactor_handwritten.npz"))
-    M = Mechanism(os.path.join(GOLDEN, "handwritten.yaml"))
     n = len(d["T0"])
     y = torch.from_numpy(np.vstack([d["T0"], d["Y0"]])).to(gpu).contiguous()
     P = torch.from_numpy(d["P"]).to(gpu)
@@ -29,7 +28,6 @@ def test_reactors_through_ignition_match_scipy_bdf(gpu):
     h = torch.full((n,), 1e-9, dtype=torch.float64, device=gpu)
     status = torch.zeros(n, dtype=torch.int32, device=gpu)
     steps = torch.zeros(n, dtype=torch.int32, device=gpu)
-    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12)
     for i, t_out in enumerate(d["times"]):
         status.zero_()
         taken = integ.advance(y, P, t, h, status, steps, float(t_out), max_steps=200000)
@@ -42,8 +40,62 @@ def test_reactors_through_ignition_match_scipy_bdf(gpu):
         dY = np.abs(got[1:] - ref[1:])
         print(f"t = {t_out:.1e} s: {taken} batch steps, max |ΔT|/T {dT.max():.2e}, max |ΔY| {dY.max():.2e}, "
               f"steps per reactor {steps.cpu().numpy().min()}..{steps.cpu().numpy().max()}", flush=True)
-        assert dT.max() <= 2e-6 and dY.max() <= 1e-6
+        assert dT.max() <= dT_max and dY.max() <= dY_max
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("jacobian,one_per_wave", [("automatic", False), ("differences", False), ("analytic", True),
+                                                   ("differences", True)])
+def test_reactors_through_ignition_match_scipy_bdf(gpu, jacobian, one_per_wave):
+    d = np.load(os.path.join(GOLDEN, "reactor_handwritten.npz"))
+    M = Mechanism(os.path.join(GOLDEN, "handwritten.yaml"))
+    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, jacobian=jacobian, one_per_wave=one_per_wave)
+    against_fixture(gpu, d, M, integ, 2e-6, 1e-6)
     assert np.all(d["states"][-1][0] > d["T0"] + 800)  # every reactor ignited
+
+
+@pytest.mark.gpu
+def test_large_mechanism_reactors_match_scipy_bdf(gpu):
+    """The 53-species mechanism (state size 54: difference quotients, one reactor per wave) against the scipy BDF
+    trajectories of tests/golden/reactor_gri53.npz (four fuel/O2/N2 mixtures to 1 ms), at the same bars."""
+    from combustion_amd import synth
+    d = np.load(os.path.join(GOLDEN, "reactor_gri53.npz"))
+    M = Mechanism(synth.mechanism_path("gri-53"))
+    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12)
+    against_fixture(gpu, d, M, integ, 2e-6, 1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["handwritten", "h2o2-10"])
+def test_jacobian_kernel_matches_the_interpreter(gpu, name):
+    """The analytic Jacobian program of the integrator (cb_build_jacobian of cb_build_reactor's program over T and
+    every Y_k) as the FP64 straight-line kernel, against the oracle's interpreter of the same program (whose
+    derivatives tests/test_jacobian.py checks against closed forms and central differences): within 1e-10 of each
+    row's largest entry (the kernel's exp and log are the device library's, the interpreter's the host's)."""
+    import torch
+
+    import oracle
+    from combustion_amd import synth
+    if name == "handwritten":
+        d = np.load(os.path.join(GOLDEN, "reactor_handwritten.npz"))
+        M = Mechanism(os.path.join(GOLDEN, "handwritten.yaml"))
+        X = np.vstack([d["states"][:, 0].reshape(-1), np.repeat(d["P"][None], 3, 0).reshape(-1),
+                       d["states"][:, 1:].transpose(1, 0, 2).reshape(M.K, -1)])
+    else:
+        M = Mechanism(synth.mechanism_path(name))
+        T, P, Y = synth.config_states(name, 509, 21)
+        X = np.vstack([T, P, Y])
+    m = M.K + 1
+    J = M.reactor().jacobian([0] + [2 + k for k in range(M.K)])
+    ref = oracle.interpret(J.linear(), X)
+    got = torch.stack(J.assemble(form="fp64")([torch.from_numpy(np.ascontiguousarray(r)).to(gpu) for r in X])).cpu().numpy()
+    np.testing.assert_array_equal(np.isfinite(got), np.isfinite(ref))
+    A, B = got[m:].reshape(m, m, -1), ref[m:].reshape(m, m, -1)
+    scale = np.max(np.abs(B), axis=1, keepdims=True) + 1e-300
+    err = np.abs(A - B) / scale
+    f_err = np.abs(got[:m] - ref[:m]) / (np.max(np.abs(ref[:m]), axis=1, keepdims=True) + 1e-300)
+    print(name, "Jacobian kernel against the interpreter: max error", err.max(), "right-hand side", f_err.max())
+    assert err.max() <= 1e-10 and f_err.max() <= 1e-10
 
 
 @pytest.mark.gpu

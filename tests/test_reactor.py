@@ -47,3 +47,20 @@ def test_reactor_conserves_mass_and_enthalpy_direction():
     Y = y[:, 1:].numpy()
     assert np.allclose(Y.sum(axis=1), 1.0, atol=1e-9)
     assert np.allclose(Y[:, -1], y0[:, -1], rtol=0, atol=1e-15)
+
+
+def test_integrator_options_are_checked_before_any_compilation():
+    """cb_integrator_create rejects flags outside cb_integrator_flags and states beyond one wave (K+1 > 64) with its
+    own errors, before it compiles or touches a device (the out == NULL call only fills the code-object cache)."""
+    from combustion_amd import Mechanism, native
+    M, mech, o = setup("h2o2-10")
+    for flags in (0x3, 0x20, 0x100):
+        with pytest.raises(native.Error) as e:
+            native._check(native.lib().cb_integrator_create(M._h, 1e-6, 1e-6, 1e-12, flags, None))
+        assert e.value.kind == "argument"
+    assert native.Integrator.flags("differences", True) == 0x12
+    assert native.Integrator.flags() == 0
+    big = Mechanism(synth.mechanism_path("large-160"))
+    with pytest.raises(native.Error) as e:
+        native.Integrator.compile(big)
+    assert e.value.kind == "unsupported"
