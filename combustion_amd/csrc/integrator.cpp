@@ -1,7 +1,7 @@
 // Batched constant-pressure reactors on the GPU: per step, f and J (the analytic Jacobian program in one kernel, or
-// difference quotients: a replicate kernel, the right-hand side over n (K+2) states and a difference kernel), the
-// three Rosenbrock phases of kernels/rosenbrock.hip and two right-hand-side evaluations, all on the caller's stream;
-// the batch is polled for reactors still running every few steps.
+// for difference quotients a replicate kernel and the right-hand side over n (K+2) states, whose outputs the first
+// Rosenbrock phase turns into J), the three Rosenbrock phases of kernels/rosenbrock.hip and two right-hand-side
+// evaluations, all on the caller's stream; the batch is polled for reactors still running every few steps.
 #include "integrator.hpp"
 
 #include <sstream>
@@ -20,6 +20,10 @@ constexpr int kPollEvery = 8;
 
 // The right-hand side program, its Jacobian program (d [dT/dt, dY/dt] / d [T, Y]; P is a parameter) and the
 // Rosenbrock kernels' source, for a mechanism.
+// One wave per block for the right-hand-side and Jacobian programs: a launch over n reactors then spreads over
+// every CU (2^14 reactors in blocks of 256 states would occupy 64 of the 256 CUs).
+constexpr unsigned kProgramBlock = 64;
+
 struct IntegratorPrograms {
   int m;
   bool differences;
@@ -44,6 +48,7 @@ struct IntegratorPrograms {
     std::ostringstream src;
     src << "#define M " << m << "\n";
     if (flags & kOneSystemPerWave) src << "#define P 1\n";
+    if (differences) src << "#define JAC_DIFFERENCES 1\n";
     src << rosenbrock_source;
     step_source = src.str();
   }
@@ -51,8 +56,8 @@ struct IntegratorPrograms {
 
 void compile_integrator(std::shared_ptr<const Mechanism> m, int flags) {
   IntegratorPrograms p(*m, flags);
-  compile_form(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, 1, 0);
-  if (!p.differences) compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, 0);
+  compile_form(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, 1, kProgramBlock);
+  if (!p.differences) compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, kProgramBlock);
   device::compile(p.step_source, "rosenbrock", hip_options(true));
 }
 
@@ -64,11 +69,11 @@ Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double a
   systems_per_wave_ = (flags & kOneSystemPerWave) ? 1u : 64u / (unsigned)m_;
   rhs_function_ = std::move(p.rhs_function);
   rhs_linear_ = std::move(p.rhs_linear);
-  rhs_ = assemble_form(ProgramView{mech_, &rhs_function_, &rhs_linear_, ProgramKind::Other, nullptr}, 1, 0);
+  rhs_ = assemble_form(ProgramView{mech_, &rhs_function_, &rhs_linear_, ProgramKind::Other, nullptr}, 1, kProgramBlock);
   if (!differences_) {
     jacobian_function_ = std::move(p.jacobian_function);
     jacobian_linear_ = std::move(p.jacobian_linear);
-    jacobian_ = assemble_form(ProgramView{mech_, &jacobian_function_, &jacobian_linear_, ProgramKind::Other, nullptr}, 1, 0);
+    jacobian_ = assemble_form(ProgramView{mech_, &jacobian_function_, &jacobian_linear_, ProgramKind::Other, nullptr}, 1, kProgramBlock);
   }
   step_.load(device::compile(p.step_source, "rosenbrock", hip_options(true)), "rosenbrock");
 }
@@ -94,13 +99,13 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
     buffers_.push_back(p);
     return p;
   };
-  jac_ = (double*)alloc((m + m * m) * n * sizeof(double));
+  jac_ = (double*)alloc((differences_ ? m : m + m * m) * n * sizeof(double));  // f(y), then J unless by differences
   F_ = (double*)alloc(m * n * sizeof(double));
   ys_ = (double*)alloc(m * n * sizeof(double));
   k1_ = (double*)alloc(m * n * sizeof(double));
   k2_ = (double*)alloc(m * n * sizeof(double));
   F1_ = (double*)alloc(m * n * sizeof(double));
-  lu_ = (double*)alloc(m * m * n * sizeof(double));
+  lu_ = (double*)alloc((m * m + m) * n * sizeof(double));  // the factors, then the reciprocal pivots
   perm_ = (int*)alloc(m * n * sizeof(int));
   running_ = (unsigned*)alloc(256);
   // pointer tables of the two programs: inputs [T, P, Y_0..Y_{K-1}] from y or ys; outputs F rows or jac rows
@@ -121,7 +126,7 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   };
   tab_y_ = table(inputs(y));
   tab_ys_ = table(inputs(ys_));
-  tab_jac_ = table(rows(jac_, m + m * m));
+  if (!differences_) tab_jac_ = table(rows(jac_, m + m * m));
   tab_F_ = table(rows(F_, m));
   if (differences_) {
     // M + 1 copies of every state: K + 2 input rows and M output rows of n (M + 1) doubles
@@ -144,15 +149,14 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   if (n == 0) return 0;
   if (n >= (size_t(1) << 31)) throw ArgumentError("at most 2^31 - 1 reactors");
   prepare(y, P, n);
-  hipFunction_t running = nullptr, clear = nullptr, replicate = nullptr, difference = nullptr;
+  hipFunction_t running = nullptr, clear = nullptr, replicate = nullptr;
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&replicate, step_.module, "rosenbrock_replicate"), "hipModuleGetFunction");
-  device::check(hipModuleGetFunction(&difference, step_.module, "rosenbrock_difference"), "hipModuleGetFunction");
   unsigned n32 = (unsigned)n;
   const unsigned systems_per_wave = systems_per_wave_;
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
-  const size_t copies = n * (size_t)(m_ + 1), columns = n * (size_t)m_;
+  const size_t copies = n * (size_t)(m_ + 1);
   long taken = 0;
   for (int step = 0; step < max_steps; step++) {
     if (differences_) {
@@ -160,14 +164,12 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
       device::check(hipModuleLaunchKernel(replicate, (unsigned)((copies + 255) / 256), 1, 1, 256, 1, 1, 0, stream, rargs, nullptr),
                     "launch rosenbrock_replicate");
       rhs_->launch_only((const double* const*)tab_rep_in_, (double* const*)tab_rep_out_, copies, stream, nullptr);
-      void* dargs[] = {&n32, &tab_rep_out_, &jac_, &y, &rtol, &atol0, &atol};
-      device::check(hipModuleLaunchKernel(difference, (unsigned)((columns + 255) / 256), 1, 1, 256, 1, 1, 0, stream, dargs, nullptr),
-                    "launch rosenbrock_difference");
     } else {
       jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
     }
     for (int phase = 1; phase <= 3; phase++) {
-      void* args[] = {&phase, &n32, &jac_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
+      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0,
+                      &atol};
       device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,
                                           nullptr),
                     "launch rosenbrock");

@@ -11,12 +11,18 @@
 // wave-uniform and broadcast with readlane; with several, every lane reads its system's pivot row with a cross-lane
 // permute (ds_bpermute), the pivot search and the error norm loop over the system's M lanes, and lanes whose system
 // is finished or past the batch compute along without storing. The LU factors and pivot order are kept in global
-// memory for phases 2 and 3. Systems whose status is not 0 (running) are left alone.
+// memory for phases 2 and 3, with the reciprocal of each row's pivot, so that the solves multiply instead of divide.
+// Systems whose status is not 0 (running) are left alone. Blocks take their systems in XCD-aware order: the 8 XCDs
+// receive blocks in turn, and block b works on system group xcd·⌊B/8⌋ + min(xcd, B mod 8) + ⌊b/8⌋ (B blocks), so
+// the systems that share a cache line of the SoA vectors run under one L2.
 // Layouts: vectors are SoA, component i of system s at [i * n + s]; jac holds f(y) in rows 0..M-1 and J row-major in
-// rows M..M+M²-1 (the Jacobian program's outputs); lu[s][i][j]; perm[s][k] = the row chosen at step k.
+// rows M..M+M²-1 (the Jacobian program's outputs); lu[s][j][i] (column-major by system) then the M reciprocal pivots
+// of system s at lu[n M² + s M + i]; perm[s][k] = the row chosen at step k.
+// With JAC_DIFFERENCES defined, J is formed in phase 1 from difference quotients instead: rosenbrock_replicate
+// writes M + 1 copies of every state (copy j + 1 with component j raised by inc_j), system by system, for one
+// right-hand-side launch over n (M + 1) states whose outputs rep_f[i][s (M + 1) + c] phase 1 reads (row i of system
+// s is M + 1 consecutive doubles), J_ij = (f_i(copy j + 1) - f_i(y)) / inc_j; phase 1 keeps f(y) in jac rows 0..M-1.
 // Required define: M; P (systems per wave) defaults to ⌊64/M⌋. Launch: ⌈n / P⌉ blocks of 64 threads.
-// With difference quotients instead of the Jacobian program, rosenbrock_replicate and rosenbrock_difference build
-// the same jac rows around one right-hand-side evaluation of n (M + 1) states.
 
 #define D_ROS 0.29289321881345248   // 1 / (2 + √2)
 #define E32 7.4142135623730950      // 6 + √2
@@ -24,6 +30,23 @@
 #ifndef P
 #define P (64 / M)  // systems per wave
 #endif
+
+// Block b's group of P systems (the XCD-aware order above).
+__device__ __forceinline__ unsigned system_group() {
+  const unsigned b = blockIdx.x, blocks = gridDim.x, xcd = b & 7u, per = blocks >> 3, rem = blocks & 7u;
+  return xcd * per + min(xcd, rem) + (b >> 3);
+}
+
+// The largest value over the wave (in every lane): DPP row shifts, then the row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ unsigned wave_max(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
 
 // The value a lane holds, for a wave-uniform lane index (two v_readlane_b32).
 __device__ __forceinline__ double lane_value(double v, int lane) {
@@ -45,8 +68,9 @@ __device__ __forceinline__ double row_value(double v, int r, int base) {
 }
 
 // Solves W x = b with the factors in registers: lane i holds row i (a[k] the multiplier l_ik for k < pos, U_i[k] for
-// k >= pos), pos the step at which row i was the pivot, perm[k] the pivot row of step k. Lane i returns x_i.
-__device__ __forceinline__ double solve(const double (&a)[M], int pos, const int (&perm)[M], double b, int i, int base) {
+// k >= pos) and rd = 1 / U_i[pos], pos the step at which row i was the pivot, perm[k] the pivot row of step k. Lane i
+// returns x_i.
+__device__ __forceinline__ double solve(const double (&a)[M], int pos, const int (&perm)[M], double rd, double b, int i, int base) {
 #pragma unroll
   for (int k = 0; k < M; k++) {  // forward: L z = P b (unit lower triangular)
     const double z = row_value(b, perm[k], base);
@@ -55,12 +79,17 @@ __device__ __forceinline__ double solve(const double (&a)[M], int pos, const int
   double x = 0.0;
 #pragma unroll
   for (int k = M - 1; k >= 0; k--) {  // backward: U x = z
-    const int p = perm[k];
-    const double xk = row_value(b, p, base) / row_value(a[k], p, base);
+    const double xk = row_value(b * rd, perm[k], base);  // row perm[k]'s b is final once the steps after k are done
     if (pos < k) b = __builtin_fma(-a[k], xk, b);
     if (i == k) x = xk;
   }
   return x;
+}
+
+// Difference quotients (the dense Jacobian of the reference's CVODE path, cvode/lib.rs:17-47): component j of a
+// system's state is increased by inc_j = √ε max(|y_j|, atol_j / rtol).
+__device__ __forceinline__ double increment(double yj, int j, double rtol, double atol0, double atol) {
+  return 1.4901161193847656e-08 * fmax(fabs(yj), (j == 0 ? atol0 : atol) / rtol);
 }
 
 // The largest value over this lane's system.
@@ -79,14 +108,15 @@ __device__ __forceinline__ double system_max(double v, int base) {
 }
 
 extern "C" __global__ void __launch_bounds__(64)
-rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* __restrict__ F, double* __restrict__ y,
+rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f, const double* __restrict__ F,
+           double* __restrict__ y,
            double* __restrict__ ys, double* __restrict__ k1, double* __restrict__ k2, double* __restrict__ F1,
            double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
            int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
   const int lane = threadIdx.x;
   const int sys = lane / M, i = lane - sys * M, base = sys * M;
   const bool row = sys < P;  // lanes past P M hold no row (with P = 1: the lanes past M)
-  const unsigned s = blockIdx.x * P + (unsigned)(row ? sys : 0);
+  const unsigned s = system_group() * P + (unsigned)(row ? sys : 0);
   const bool valid = row && s < n;
   const unsigned sv = valid ? s : 0u;  // an in-bounds system for the loads of lanes that store nothing
 #if P == 1
@@ -110,6 +140,7 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
 #endif
   int* perm_s = perm_all + (size_t)sv * M;
   double* lu_s = lu + (size_t)sv * M * M;
+  double* rd_s = lu + (size_t)n * M * M + (size_t)sv * M;
   double a[M];
   int perm[M];  // the same in every lane of a system (wave-uniform when P = 1)
   int pos = -1;
@@ -117,25 +148,46 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
 
   if (phase == 1) {
     // W = I - h d J, factored with partial pivoting
+#ifdef JAC_DIFFERENCES
+    const double* Fi = rep_f[ir] + (size_t)sv * (M + 1);  // row i of the system's copies
+    const double f0 = row ? Fi[0] : 0.0;
+    const double rinc_i = row ? 1.0 / increment(y[(size_t)ir * n + sv], i, rtol, atol0, atol) : 0.0;
+    if (active) jac[(size_t)i * n + s] = f0;  // f(y), for phase 3
+#else
+    const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
+#endif
 #pragma unroll
-    for (int j = 0; j < M; j++) a[j] = (row ? -hs * D_ROS * jac[(size_t)(M + ir * M + j) * n + sv] : 0.0) + (row && i == j ? 1.0 : 0.0);
+    for (int j = 0; j < M; j++) {
+#ifdef JAC_DIFFERENCES
+      const double Jij = (Fi[j + 1] - f0) * row_value(rinc_i, j, base);
+#else
+      const double Jij = jac[(size_t)(M + ir * M + j) * n + sv];
+#endif
+      a[j] = (row ? -hs * D_ROS * Jij : 0.0) + (row && i == j ? 1.0 : 0.0);
+    }
+    double diag = 1.0;  // this row's pivot, U_i[pos]
 #pragma unroll
     for (int k = 0; k < M; k++) {
       // pivot: the largest |a[k]| over the rows not yet chosen (ties to the lowest row)
       const double key = row && pos < 0 ? fabs(a[k]) : -1.0;
 #if P == 1
-      double best = key;
-      int who = lane;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ok = __shfl_xor(best, o);
-        const int ow = __shfl_xor(who, o);
-        if (ok > best || (ok == best && ow < who)) {
-          best = ok;
-          who = ow;
+      // the high words of |a| (plus one; 0 for rows already chosen) order the rows as |a| does up to ties, which
+      // the rare case resolves over the tied lanes in ascending order
+      const unsigned hi = row && pos < 0 ? (unsigned)(__builtin_bit_cast(unsigned long long, key) >> 32) + 1u : 0u;
+      unsigned long long tied = __builtin_amdgcn_ballot_w64(hi == wave_max(hi));
+      int p = (int)__builtin_ctzll(tied);
+      if (tied & (tied - 1)) {
+        double best = -1.0;
+        while (tied) {
+          const int l = (int)__builtin_ctzll(tied);
+          tied &= tied - 1;
+          const double v = lane_value(key, l);
+          if (v > best) {
+            best = v;
+            p = l;
+          }
         }
       }
-      const int p = __builtin_amdgcn_readfirstlane(who);
 #else
       double best = -2.0;
       int p = 0;
@@ -153,21 +205,23 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
       if (row && i == p) pos = k;
       // the pivot row's entries are read from its lane by every lane (outside the branch: the pivot row is final)
       const double pivot = row_value(a[k], p, base);
+      if (row && i == p) diag = pivot;
       const bool eliminate = row && pos < 0;
       const double l = eliminate ? a[k] / pivot : 0.0;
       if (eliminate) a[k] = l;
 #pragma unroll
       for (int j = k + 1; j < M; j++) {
         const double pj = row_value(a[j], p, base);
-        if (eliminate) a[j] = __builtin_fma(-l, pj, a[j]);
+        a[j] = __builtin_fma(-l, pj, a[j]);  // l = 0 leaves the rows already chosen as they are (finite pj)
       }
     }
+    const double rd = 1.0 / diag;
     if (active) {
 #pragma unroll
-      for (int j = 0; j < M; j++) lu_s[i * M + j] = a[j];
+      for (int j = 0; j < M; j++) lu_s[j * M + i] = a[j];
+      rd_s[i] = rd;
     }
-    const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
-    const double x = solve(a, pos, perm, f0, i, base);
+    const double x = solve(a, pos, perm, rd, f0, i, base);
     if (active) {
       k1[(size_t)i * n + s] = x;
       ys[(size_t)i * n + s] = y[(size_t)i * n + s] + 0.5 * hs * x;
@@ -177,7 +231,8 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
 
   // phases 2 and 3: the factors and each row's pivot step back from memory
 #pragma unroll
-  for (int j = 0; j < M; j++) a[j] = row ? lu_s[ir * M + j] : 0.0;
+  for (int j = 0; j < M; j++) a[j] = row ? lu_s[j * M + ir] : 0.0;
+  const double rd = row ? rd_s[ir] : 1.0;
 #pragma unroll
   for (int k = 0; k < M; k++) {
 #if P == 1
@@ -190,7 +245,7 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
   const double x1 = row ? k1[(size_t)ir * n + sv] : 0.0;
   if (phase == 2) {
     const double f1 = row ? F[(size_t)ir * n + sv] : 0.0;
-    const double x = solve(a, pos, perm, f1 - x1, i, base) + x1;
+    const double x = solve(a, pos, perm, rd, f1 - x1, i, base) + x1;
     if (active) {
       F1[(size_t)i * n + s] = f1;
       k2[(size_t)i * n + s] = x;
@@ -202,7 +257,7 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
   const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
   const double f1 = row ? F1[(size_t)ir * n + sv] : 0.0;
   const double f2 = row ? F[(size_t)ir * n + sv] : 0.0;
-  const double x3 = solve(a, pos, perm, f2 - E32 * (x2 - f1) - 2.0 * (x1 - f0), i, base);
+  const double x3 = solve(a, pos, perm, rd, f2 - E32 * (x2 - f1) - 2.0 * (x1 - f0), i, base);
   const double y0 = row ? y[(size_t)ir * n + sv] : 0.0, y1 = row ? ys[(size_t)ir * n + sv] : 0.0;
   const double err = hs * (x1 - 2.0 * x2 + x3) * (1.0 / 6.0);
   const double scale = (i == 0 ? atol0 : atol) + rtol * fmax(fabs(y0), fabs(y1));
@@ -232,42 +287,20 @@ rosenbrock(int phase, unsigned n, const double* __restrict__ jac, const double* 
   }
 }
 
-// Difference quotients (the dense Jacobian of the reference's CVODE path, cvode/lib.rs:17-47): component j of a
-// system's state is increased by inc_j = √ε max(|y_j|, atol_j / rtol).
-__device__ __forceinline__ double increment(double yj, int j, double rtol, double atol0, double atol) {
-  return 1.4901161193847656e-08 * fmax(fabs(yj), (j == 0 ? atol0 : atol) / rtol);
-}
-
 // Copies c = 0..M of every system's state as inputs [T, P, Y_0..Y_{M-2}] of the right-hand-side program over n (M + 1)
-// states (copy c at [c n, (c + 1) n)): copy 0 unchanged, copy j + 1 with component j increased by inc_j.
+// states (copy c of system s at s (M + 1) + c): copy 0 unchanged, copy j + 1 with component j increased by inc_j.
 extern "C" __global__ void __launch_bounds__(256)
 rosenbrock_replicate(unsigned n, const double* __restrict__ y, const double* __restrict__ P_in, double* const* __restrict__ rep,
                      double rtol, double atol0, double atol) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)n * (M + 1)) return;
-  const unsigned c = (unsigned)(idx / n), s = (unsigned)(idx - (size_t)c * n);
+  const unsigned s = (unsigned)(idx / (M + 1)), c = (unsigned)(idx - (size_t)s * (M + 1));
   for (int j = 0; j < M; j++) {
     double v = y[(size_t)j * n + s];
     if ((int)c == j + 1) v += increment(v, j, rtol, atol0, atol);
     rep[j == 0 ? 0 : j + 1][idx] = v;
   }
   rep[1][idx] = P_in[s];
-}
-
-// jac rows 0..M-1 = f(y) (copy 0), rows M + i M + j = (f_i(copy j + 1) - f_i(y)) / inc_j; one thread per (j, system).
-extern "C" __global__ void __launch_bounds__(256)
-rosenbrock_difference(unsigned n, const double* const* __restrict__ F, double* __restrict__ jac, const double* __restrict__ y,
-                      double rtol, double atol0, double atol) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n * M) return;
-  const int j = (int)(idx / n);
-  const unsigned s = (unsigned)(idx - (size_t)j * n);
-  const double rinc = 1.0 / increment(y[(size_t)j * n + s], j, rtol, atol0, atol);
-  for (int i = 0; i < M; i++) {
-    const double f0 = F[i][s];
-    if (j == 0) jac[(size_t)i * n + s] = f0;
-    jac[(size_t)(M + i * M + j) * n + s] = (F[i][(size_t)(j + 1) * n + s] - f0) * rinc;
-  }
 }
 
 // The number of systems still running (status 0), added to *count (cleared by the caller's launch of clear_count).

@@ -196,7 +196,7 @@ enum cb_integrator_flags {
   CB_JACOBIAN_ANALYTIC = 1,    /* cb_build_jacobian's forward-mode program, f and J in one straight-line kernel */
   CB_JACOBIAN_DIFFERENCES = 2, /* difference quotients, as CVODE's dense solver forms them: every state copied K+2
                                   times (one copy per perturbed component, inc_j = sqrt(eps) max(|y_j|, atol_j/rtol)),
-                                  one right-hand-side launch over the copies, one difference kernel */
+                                  one right-hand-side launch over the copies, the quotients formed by the LU kernel */
   CB_INTEGRATOR_ONE_PER_WAVE = 0x10 /* one reactor per Rosenbrock wave whatever its size (otherwise floor(64/(K+1))) */
 };
 int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, int32_t flags, cb_integrator** out);
