@@ -277,4 +277,92 @@ int32_t peak_live(const Linear& l) {
   return peak;
 }
 
+namespace {
+// The registers an instruction reads, by arity: none for a constant, one for Neg/Sqrt/Exp/Ln/Sq, three for Select.
+int operands(const Instruction& i, int32_t (&r)[3]) {
+  if (i.op == LOp::Const) return 0;
+  r[0] = i.a;
+  if (i.op == LOp::Neg || i.op == LOp::Sqrt || i.op == LOp::Exp || i.op == LOp::Ln || i.op == LOp::Sq) return 1;
+  r[1] = i.b;
+  if (i.op != LOp::Select) return 2;
+  r[2] = i.c;
+  return 3;
+}
+}  // namespace
+
+Split split_at(const Linear& l, int32_t first_varying) {
+  if (first_varying < 0 || first_varying > l.n_input) throw Error("split: the first varying input is out of range");
+  const int32_t R = l.n_registers;
+  std::vector<char> varies(R, 0), constant(R, 0);
+  std::vector<const Instruction*> def(R, nullptr);
+  for (int32_t r = first_varying; r < l.n_input; r++) varies[r] = 1;
+  int32_t ops[3];
+  for (const Instruction& i : l.code) {
+    def[i.dst] = &i;
+    if (i.op == LOp::Const) constant[i.dst] = 1;
+    const int n = operands(i, ops);
+    for (int k = 0; k < n; k++) varies[i.dst] |= varies[ops[k]];
+  }
+  // Shared values the varying part reads (neither inputs nor constants), in program order.
+  std::vector<int32_t> boundary_index(R, -1), boundary;
+  auto read = [&](int32_t r) {
+    if (varies[r] || constant[r] || r < l.n_input || boundary_index[r] >= 0) return;
+    boundary_index[r] = (int32_t)boundary.size();
+    boundary.push_back(r);
+  };
+  for (const Instruction& i : l.code) {
+    if (!varies[i.dst]) continue;
+    const int n = operands(i, ops);
+    for (int k = 0; k < n; k++) read(ops[k]);
+  }
+  for (int32_t r : l.output) read(r);
+  std::sort(boundary.begin(), boundary.end());
+  for (size_t b = 0; b < boundary.size(); b++) boundary_index[boundary[b]] = (int32_t)b;
+
+  Split s;
+  // shared: the instructions the boundary values need, in program order, on the original registers
+  std::vector<char> needed(R, 0);
+  for (int32_t r : boundary) needed[r] = 1;
+  for (auto it = l.code.rbegin(); it != l.code.rend(); ++it) {
+    if (!needed[it->dst]) continue;
+    const int n = operands(*it, ops);
+    for (int k = 0; k < n; k++) needed[ops[k]] = 1;
+  }
+  s.shared.n_input = first_varying;
+  s.shared.n_registers = R;
+  for (const Instruction& i : l.code)
+    if (needed[i.dst]) s.shared.code.push_back(i);
+  s.shared.output = boundary;
+
+  // varying: inputs, then the boundary values, then its own instructions renumbered in program order
+  const int32_t n_in = l.n_input + (int32_t)boundary.size();
+  std::vector<int32_t> map(R, -1);
+  for (int32_t r = 0; r < l.n_input; r++) map[r] = r;
+  for (size_t b = 0; b < boundary.size(); b++) map[boundary[b]] = l.n_input + (int32_t)b;
+  int32_t next = n_in;
+  auto operand = [&](int32_t r) {
+    if (map[r] < 0) {  // a constant read for the first time: emitted again here
+      if (!constant[r]) throw Error("split: an operand is neither an input, a shared value nor a constant");
+      Instruction c = *def[r];
+      c.dst = map[r] = next++;
+      s.varying.code.push_back(c);
+    }
+    return map[r];
+  };
+  for (const Instruction& i : l.code) {
+    if (!varies[i.dst]) continue;
+    Instruction v = i;
+    const int n = operands(i, ops);
+    if (n > 0) v.a = operand(i.a);
+    if (n > 1) v.b = operand(i.b);
+    if (n > 2) v.c = operand(i.c);
+    v.dst = map[i.dst] = next++;
+    s.varying.code.push_back(v);
+  }
+  for (int32_t r : l.output) s.varying.output.push_back(operand(r));
+  s.varying.n_input = n_in;
+  s.varying.n_registers = next;
+  return s;
+}
+
 }  // namespace combustion::ast

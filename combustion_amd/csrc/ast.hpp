@@ -116,4 +116,14 @@ OpCounts count(const Linear& l);
 // kernel before any rescheduling.
 int32_t peak_live(const Linear& l);
 
+// A program split at its dependence on the inputs from `first_varying` on: `shared` computes from the inputs before
+// it (its own inputs, the same registers) the B values that the varying part reads (its outputs); `varying` takes
+// the program's inputs followed by those B values (registers n_input..n_input+B-1) and computes the program's
+// outputs. Constants the varying part reads are emitted in it again. Evaluating `shared` once and `varying` for many
+// inputs that differ only from `first_varying` on gives, instruction for instruction, the program's results.
+struct Split {
+  Linear shared, varying;
+};
+Split split_at(const Linear& l, int32_t first_varying);
+
 }  // namespace combustion::ast

@@ -351,6 +351,26 @@ int cb_build_jacobian(const cb_program* p, int32_t n_wrt, const int32_t* wrt, in
   });
 }
 
+int cb_program_split(const cb_program* p, int32_t first_varying, cb_program** shared, cb_program** varying) {
+  return guard([&] {
+    require(p && shared && varying, "null argument");
+    require(first_varying >= 0 && first_varying <= p->linear.n_input, "first_varying out of range");
+    ast::Split parts = ast::split_at(p->linear, first_varying);
+    auto wrap = [&](ast::Linear l) {
+      auto* q = new cb_program;
+      q->mechanism = p->mechanism;
+      q->function.n_input = l.n_input;
+      q->function.output.assign(l.output.size(), -1);  // the program lives in its linear form only
+      q->linear = std::move(l);
+      q->kind = ProgramKind::Other;
+      return q;
+    };
+    std::unique_ptr<cb_program> s(wrap(std::move(parts.shared)));
+    *varying = wrap(std::move(parts.varying));
+    *shared = s.release();
+  });
+}
+
 void cb_program_free(cb_program* p) { delete p; }
 
 int cb_program_sizes(const cb_program* p, int32_t* inputs, int32_t* outputs, int64_t* statements, int64_t* instructions, int32_t* registers) {

@@ -82,6 +82,14 @@ void compile_form(const ProgramView& p, int form, unsigned block_size) {
   device::compile(emit_straight(*p.linear, "program", straight_options(form, block_size)), "program", hip_options(form != 0));
 }
 
+void compile_straight(const ProgramView& p, const StraightOptions& o) {
+  device::compile(emit_straight(*p.linear, "program", o), "program", hip_options(true));
+}
+
+std::unique_ptr<Form> assemble_straight(const ProgramView& p, const StraightOptions& o) {
+  return std::make_unique<Straight>(p, o, true);
+}
+
 std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size) {
   check_form(form);
   if (use_tiled(p, form)) return assemble_tiled(p, form, block_size);

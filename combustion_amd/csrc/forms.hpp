@@ -77,6 +77,12 @@ std::string form_source(const ProgramView& p, int form, unsigned block_size);
 void compile_form(const ProgramView& p, int form, unsigned block_size);
 std::unique_ptr<Form> assemble_form(const ProgramView& p, int form, unsigned block_size);
 
+// A straight-line program with explicit emission options (LDS-resident registers, say), FP contraction on.
+struct StraightOptions;
+StraightOptions straight_options(int form, unsigned block_size);
+void compile_straight(const ProgramView& p, const StraightOptions& o);
+std::unique_ptr<Form> assemble_straight(const ProgramView& p, const StraightOptions& o);
+
 // Tiled forms (tiled.cpp).
 bool tiled_applies(const ProgramView& p);
 // form: 1 FP64, 2 MFMA mixed precision, 3 FP32-core straight-line rates (see cb_form), with kFormTiledTransport.

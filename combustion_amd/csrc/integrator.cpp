@@ -1,7 +1,9 @@
 // Batched constant-pressure reactors on the GPU: per step, f and J (the analytic Jacobian program in one kernel, or
-// for difference quotients a replicate kernel and the right-hand side over n (K+2) states, whose outputs the first
-// Rosenbrock phase turns into J), the three Rosenbrock phases of kernels/rosenbrock.hip and two right-hand-side
-// evaluations, all on the caller's stream; the batch is polled for reactors still running every few steps.
+// difference quotients: a replicate kernel, the right-hand side split at the mass fractions — its temperature and
+// pressure part once per reactor, the rest over the reactor's K+1 copies — and the whole right-hand side at the raised
+// temperature, whose outputs the first Rosenbrock phase turns into J), the three Rosenbrock phases of
+// kernels/rosenbrock.hip and two right-hand-side evaluations, all on the caller's stream; the batch is polled for
+// reactors still running every few steps.
 #include "integrator.hpp"
 
 #include <sstream>
@@ -24,31 +26,80 @@ constexpr int kPollEvery = 8;
 // every CU (2^14 reactors in blocks of 256 states would occupy 64 of the 256 CUs).
 constexpr unsigned kProgramBlock = 64;
 
+// The right-hand side as the FP64 rates kernel is built: the lean program (each rate-of-progress product started
+// from its reaction's rate constant, so that fewer values stay live), and, when its peak live set exceeds two waves'
+// registers, its clamped exp(-G_k/RT) values in LDS (at K=53: no scratch, against 1.2 KB per lane for the plain
+// program in registers).
+StraightOptions rhs_options(const ast::Linear& l) {
+  StraightOptions o = straight_options(1, kProgramBlock);
+  if (ast::peak_live(l) > 128) o.lds_registers = gibbs_exponentials(l, true);
+  if (o.lds_registers.size() * o.block_size * 8 > 160 * 1024) o.lds_registers.clear();
+  return o;
+}
+
+// Difference quotients: the copies of a reactor, K+1 rounded up to a power of two (the copies past K+1 repeat the
+// state), so that a block of 64 holds whole reactors.
+unsigned copies_per_reactor(int m) {
+  unsigned c = 1;
+  while (c < (unsigned)m) c <<= 1;
+  return c;
+}
+
+// The part of the right-hand side shared by a reactor's copies (from T and P), its values stored reactor by reactor
+// (the B values of reactor s at [s B, (s + 1) B)).
+StraightOptions shared_options(const ast::Linear& shared) {
+  StraightOptions o = straight_options(1, kProgramBlock);
+  o.output_stride = (unsigned)shared.output.size();
+  return o;
+}
+
+// The part that varies with the mass fractions, over the copies: a block of 64 holds whole reactors, whose shared
+// values it stages in LDS.
+StraightOptions varying_options(const ast::Linear& varying, int m) {
+  StraightOptions o = rhs_options(varying);
+  o.group_from = m + 1;  // after the inputs T, P, Y_0..Y_{K-1}
+  o.group_size = copies_per_reactor(m);
+  o.group_packed = true;
+  return o;
+}
+
+ast::Function placeholder(const ast::Linear& l) {  // the Function of a program that lives in its linear form only
+  ast::Function f;
+  f.n_input = l.n_input;
+  f.output.assign(l.output.size(), -1);
+  return f;
+}
+
 struct IntegratorPrograms {
   int m;
   bool differences;
-  ast::Function rhs_function, jacobian_function;
-  ast::Linear rhs_linear, jacobian_linear;
+  ast::Function rhs_function, jacobian_function, shared_function, varying_function;
+  ast::Linear rhs_linear, jacobian_linear, shared_linear, varying_linear;
   std::string step_source;
   IntegratorPrograms(const Mechanism& mech, int flags) : m((int)mech.species.len() + 1) {
     if (m > 64) throw Unsupported("the batched integrator holds a reactor's state in one wave: at most 63 species");
     const int jac = flags & kJacobianMask;
     if ((flags & ~(kJacobianMask | kOneSystemPerWave)) || jac > kJacobianDifferences) throw ArgumentError("unknown integrator flags");
     differences = jac == kJacobianDifferences || (jac == kJacobianAutomatic && m > kAnalyticMaxState);
-    rhs_function = reactor(mech, false);
+    rhs_function = reactor(mech, true);
     rhs_linear = ast::linearize(rhs_function);
     if (!differences) {
       std::vector<int32_t> wrt = {0}, rows;
       for (int k = 0; k + 1 < m; k++) wrt.push_back(2 + k);
       for (int o = 0; o < m; o++) rows.push_back(o);
       jacobian_linear = ast::jacobian(rhs_linear, wrt, rows);
-      jacobian_function.n_input = jacobian_linear.n_input;
-      jacobian_function.output.assign(jacobian_linear.output.size(), -1);
+      jacobian_function = placeholder(jacobian_linear);
+    } else {
+      ast::Split parts = ast::split_at(rhs_linear, 2);  // T and P shared, the mass fractions varying
+      shared_linear = std::move(parts.shared);
+      varying_linear = std::move(parts.varying);
+      shared_function = placeholder(shared_linear);
+      varying_function = placeholder(varying_linear);
     }
     std::ostringstream src;
     src << "#define M " << m << "\n";
     if (flags & kOneSystemPerWave) src << "#define P 1\n";
-    if (differences) src << "#define JAC_DIFFERENCES 1\n";
+    if (differences) src << "#define JAC_DIFFERENCES 1\n#define COPIES " << copies_per_reactor(m) << "\n";
     src << rosenbrock_source;
     step_source = src.str();
   }
@@ -56,8 +107,13 @@ struct IntegratorPrograms {
 
 void compile_integrator(std::shared_ptr<const Mechanism> m, int flags) {
   IntegratorPrograms p(*m, flags);
-  compile_form(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, 1, kProgramBlock);
+  compile_straight(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, rhs_options(p.rhs_linear));
   if (!p.differences) compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, kProgramBlock);
+  if (p.differences) {
+    compile_straight(ProgramView{m, &p.shared_function, &p.shared_linear, ProgramKind::Other, nullptr}, shared_options(p.shared_linear));
+    compile_straight(ProgramView{m, &p.varying_function, &p.varying_linear, ProgramKind::Other, nullptr},
+                     varying_options(p.varying_linear, p.m));
+  }
   device::compile(p.step_source, "rosenbrock", hip_options(true));
 }
 
@@ -69,11 +125,20 @@ Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double a
   systems_per_wave_ = (flags & kOneSystemPerWave) ? 1u : 64u / (unsigned)m_;
   rhs_function_ = std::move(p.rhs_function);
   rhs_linear_ = std::move(p.rhs_linear);
-  rhs_ = assemble_form(ProgramView{mech_, &rhs_function_, &rhs_linear_, ProgramKind::Other, nullptr}, 1, kProgramBlock);
+  rhs_ = assemble_straight(ProgramView{mech_, &rhs_function_, &rhs_linear_, ProgramKind::Other, nullptr}, rhs_options(rhs_linear_));
   if (!differences_) {
     jacobian_function_ = std::move(p.jacobian_function);
     jacobian_linear_ = std::move(p.jacobian_linear);
     jacobian_ = assemble_form(ProgramView{mech_, &jacobian_function_, &jacobian_linear_, ProgramKind::Other, nullptr}, 1, kProgramBlock);
+  } else {
+    shared_function_ = std::move(p.shared_function);
+    shared_linear_ = std::move(p.shared_linear);
+    varying_function_ = std::move(p.varying_function);
+    varying_linear_ = std::move(p.varying_linear);
+    shared_ = assemble_straight(ProgramView{mech_, &shared_function_, &shared_linear_, ProgramKind::Other, nullptr},
+                                shared_options(shared_linear_));
+    varying_ = assemble_straight(ProgramView{mech_, &varying_function_, &varying_linear_, ProgramKind::Other, nullptr},
+                                 varying_options(varying_linear_, m_));
   }
   step_.load(device::compile(p.step_source, "rosenbrock", hip_options(true)), "rosenbrock");
 }
@@ -129,15 +194,29 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   if (!differences_) tab_jac_ = table(rows(jac_, m + m * m));
   tab_F_ = table(rows(F_, m));
   if (differences_) {
-    // M + 1 copies of every state: K + 2 input rows and M output rows of n (M + 1) doubles
-    const size_t nr = n * (m + 1);
+    // the shared values (B per reactor, reactor by reactor); 64 copies of every state (K + 2 input rows and M output
+    // rows of 64 n doubles, the input table followed by the shared array); the temperature-raised states (T row,
+    // then P and y's rows)
+    const size_t B = shared_linear_.output.size(), nr = n * copies_per_reactor(m_);
+    double* shared = (double*)alloc(B * n * sizeof(double));
     double* rep_in = (double*)alloc((m + 1) * nr * sizeof(double));
     double* rep_out = (double*)alloc(m * nr * sizeof(double));
+    double* t_out = (double*)alloc(m * n * sizeof(double));
+    raised_T_ = (double*)alloc(n * sizeof(double));
     std::vector<const void*> in_rows, out_rows;
     for (size_t i = 0; i < m + 1; i++) in_rows.push_back(rep_in + i * nr);
+    in_rows.push_back(shared);
     for (size_t i = 0; i < m; i++) out_rows.push_back(rep_out + i * nr);
+    std::vector<const void*> shared_out;
+    for (size_t b = 0; b < B; b++) shared_out.push_back(shared + b);
+    tab_shared_in_ = table({y, P});
+    tab_shared_out_ = table(shared_out);
     tab_rep_in_ = table(in_rows);
     tab_rep_out_ = table(out_rows);
+    std::vector<const void*> t_in = inputs(y);
+    t_in[0] = raised_T_;
+    tab_t_in_ = table(t_in);
+    tab_t_out_ = table(rows(t_out, m));
   }
   n_ = n;
   last_y_ = y;
@@ -152,23 +231,25 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   hipFunction_t running = nullptr, clear = nullptr, replicate = nullptr;
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
-  device::check(hipModuleGetFunction(&replicate, step_.module, "rosenbrock_replicate"), "hipModuleGetFunction");
+  if (differences_) device::check(hipModuleGetFunction(&replicate, step_.module, "rosenbrock_replicate"), "hipModuleGetFunction");
   unsigned n32 = (unsigned)n;
   const unsigned systems_per_wave = systems_per_wave_;
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
-  const size_t copies = n * (size_t)(m_ + 1);
+  const size_t copies = n * (size_t)copies_per_reactor(m_);
   long taken = 0;
   for (int step = 0; step < max_steps; step++) {
     if (differences_) {
-      void* rargs[] = {&n32, &y, &P, &tab_rep_in_, &rtol, &atol0, &atol};
+      void* rargs[] = {&n32, &y, &P, &tab_rep_in_, &raised_T_, &rtol, &atol0, &atol};
       device::check(hipModuleLaunchKernel(replicate, (unsigned)((copies + 255) / 256), 1, 1, 256, 1, 1, 0, stream, rargs, nullptr),
                     "launch rosenbrock_replicate");
-      rhs_->launch_only((const double* const*)tab_rep_in_, (double* const*)tab_rep_out_, copies, stream, nullptr);
+      shared_->launch_only((const double* const*)tab_shared_in_, (double* const*)tab_shared_out_, n, stream, nullptr);
+      varying_->launch_only((const double* const*)tab_rep_in_, (double* const*)tab_rep_out_, copies, stream, nullptr);
+      rhs_->launch_only((const double* const*)tab_t_in_, (double* const*)tab_t_out_, n, stream, nullptr);
     } else {
       jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
     }
     for (int phase = 1; phase <= 3; phase++) {
-      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0,
+      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0,
                       &atol};
       device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,
                                           nullptr),

@@ -17,7 +17,7 @@ namespace combustion {
 // Options (cb_integrator_create's flags): the Jacobian, and the Rosenbrock kernel's packing.
 constexpr int kJacobianAutomatic = 0;    // the analytic program up to kAnalyticMaxState rows, difference quotients above
 constexpr int kJacobianAnalytic = 1;     // the forward-mode program of cb_build_jacobian
-constexpr int kJacobianDifferences = 2;  // difference quotients around one right-hand-side launch of n (K+2) states
+constexpr int kJacobianDifferences = 2;  // difference quotients from n (K+1) copies of the states (split right-hand side)
 constexpr int kJacobianMask = 0xF;
 constexpr int kOneSystemPerWave = 0x10;  // one reactor per Rosenbrock wave (readlane broadcasts) whatever its size
 constexpr int kAnalyticMaxState = 8;  // beyond, the analytic program spills (3.2 KB per lane at 11 rows) and loses to differences
@@ -45,9 +45,9 @@ class Integrator {
   double rtol_, atol_T_, atol_Y_;
   bool differences_;
   unsigned systems_per_wave_;
-  ast::Function rhs_function_, jacobian_function_;
-  ast::Linear rhs_linear_, jacobian_linear_;
-  std::unique_ptr<Form> rhs_, jacobian_;
+  ast::Function rhs_function_, jacobian_function_, shared_function_, varying_function_;
+  ast::Linear rhs_linear_, jacobian_linear_, shared_linear_, varying_linear_;
+  std::unique_ptr<Form> rhs_, jacobian_, shared_, varying_;
   device::Kernel step_;
   size_t n_ = 0;
   std::vector<void*> buffers_;
@@ -55,7 +55,11 @@ class Integrator {
   int* perm_ = nullptr;
   unsigned* running_ = nullptr;
   void **tab_y_ = nullptr, **tab_ys_ = nullptr, **tab_jac_ = nullptr, **tab_F_ = nullptr;
-  void **tab_rep_in_ = nullptr, **tab_rep_out_ = nullptr;  // difference quotients: the n (K+2) states and their f
+  // difference quotients: the shared part's inputs [T, P] and outputs; the copies' inputs (followed by the shared
+  // values) and outputs; the temperature-raised states' inputs and outputs; their raised temperatures
+  void **tab_shared_in_ = nullptr, **tab_shared_out_ = nullptr, **tab_rep_in_ = nullptr, **tab_rep_out_ = nullptr;
+  void **tab_t_in_ = nullptr, **tab_t_out_ = nullptr;
+  double* raised_T_ = nullptr;
   const double* last_y_ = nullptr;
   const double* last_P_ = nullptr;
   void release();

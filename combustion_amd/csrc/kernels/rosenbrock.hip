@@ -18,10 +18,12 @@
 // Layouts: vectors are SoA, component i of system s at [i * n + s]; jac holds f(y) in rows 0..M-1 and J row-major in
 // rows M..M+M²-1 (the Jacobian program's outputs); lu[s][j][i] (column-major by system) then the M reciprocal pivots
 // of system s at lu[n M² + s M + i]; perm[s][k] = the row chosen at step k.
-// With JAC_DIFFERENCES defined, J is formed in phase 1 from difference quotients instead: rosenbrock_replicate
-// writes M + 1 copies of every state (copy j + 1 with component j raised by inc_j), system by system, for one
-// right-hand-side launch over n (M + 1) states whose outputs rep_f[i][s (M + 1) + c] phase 1 reads (row i of system
-// s is M + 1 consecutive doubles), J_ij = (f_i(copy j + 1) - f_i(y)) / inc_j; phase 1 keeps f(y) in jac rows 0..M-1.
+// With JAC_DIFFERENCES defined (and COPIES, a power of two >= M), J is formed in phase 1 from difference quotients
+// instead: rosenbrock_replicate writes COPIES copies of every state, system by system (copy 0 and copies past M-1 the
+// state, copy j = 1..M-1 with component j raised by inc_j), and the state's temperature raised by inc_0; phase 1
+// reads the right-hand sides of the copies, rep_f[i][COPIES s + c] (row i of system s is COPIES consecutive
+// doubles), and at the raised temperature, rep_t[i][s]: J_ij = (f_i(copy j) - f_i(y)) / inc_j,
+// J_i0 = (f_i(raised T) - f_i(y)) / inc_0; f(y) goes to jac rows 0..M-1.
 // Required define: M; P (systems per wave) defaults to ⌊64/M⌋. Launch: ⌈n / P⌉ blocks of 64 threads.
 
 #define D_ROS 0.29289321881345248   // 1 / (2 + √2)
@@ -108,7 +110,8 @@ __device__ __forceinline__ double system_max(double v, int base) {
 }
 
 extern "C" __global__ void __launch_bounds__(64)
-rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f, const double* __restrict__ F,
+rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+           const double* const* __restrict__ rep_t, const double* __restrict__ F,
            double* __restrict__ y,
            double* __restrict__ ys, double* __restrict__ k1, double* __restrict__ k2, double* __restrict__ F1,
            double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
@@ -149,8 +152,9 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
   if (phase == 1) {
     // W = I - h d J, factored with partial pivoting
 #ifdef JAC_DIFFERENCES
-    const double* Fi = rep_f[ir] + (size_t)sv * (M + 1);  // row i of the system's copies
+    const double* Fi = rep_f[ir] + (size_t)sv * COPIES;  // row i of the system's copies
     const double f0 = row ? Fi[0] : 0.0;
+    const double fT = row ? rep_t[ir][sv] : 0.0;  // row i at the raised temperature
     const double rinc_i = row ? 1.0 / increment(y[(size_t)ir * n + sv], i, rtol, atol0, atol) : 0.0;
     if (active) jac[(size_t)i * n + s] = f0;  // f(y), for phase 3
 #else
@@ -159,7 +163,7 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
 #pragma unroll
     for (int j = 0; j < M; j++) {
 #ifdef JAC_DIFFERENCES
-      const double Jij = (Fi[j + 1] - f0) * row_value(rinc_i, j, base);
+      const double Jij = ((j == 0 ? fT : Fi[j]) - f0) * row_value(rinc_i, j, base);
 #else
       const double Jij = jac[(size_t)(M + ir * M + j) * n + sv];
 #endif
@@ -287,21 +291,28 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
   }
 }
 
-// Copies c = 0..M of every system's state as inputs [T, P, Y_0..Y_{M-2}] of the right-hand-side program over n (M + 1)
-// states (copy c of system s at s (M + 1) + c): copy 0 unchanged, copy j + 1 with component j increased by inc_j.
+#ifdef JAC_DIFFERENCES
+// Copies c = 0..COPIES-1 of every system's state as inputs [T, P, Y_0..Y_{M-2}] of the mass-fraction part of the
+// right-hand side over COPIES n states (copy c of system s at COPIES s + c): copy j = 1..M-1 with component j (Y_{j-1})
+// increased by inc_j, the others unchanged; and raised_T[s] = T + inc_0 for the whole right-hand side at the raised
+// temperature.
 extern "C" __global__ void __launch_bounds__(256)
 rosenbrock_replicate(unsigned n, const double* __restrict__ y, const double* __restrict__ P_in, double* const* __restrict__ rep,
-                     double rtol, double atol0, double atol) {
+                     double* __restrict__ raised_T, double rtol, double atol0, double atol) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n * (M + 1)) return;
-  const unsigned s = (unsigned)(idx / (M + 1)), c = (unsigned)(idx - (size_t)s * (M + 1));
-  for (int j = 0; j < M; j++) {
-    double v = y[(size_t)j * n + s];
-    if ((int)c == j + 1) v += increment(v, j, rtol, atol0, atol);
-    rep[j == 0 ? 0 : j + 1][idx] = v;
-  }
+  if (idx >= (size_t)n * COPIES) return;
+  const unsigned s = (unsigned)(idx / COPIES), c = (unsigned)(idx % COPIES);
+  const double T = y[s];
+  rep[0][idx] = T;
   rep[1][idx] = P_in[s];
+  for (int j = 1; j < M; j++) {
+    double v = y[(size_t)j * n + s];
+    if ((int)c == j) v += increment(v, j, rtol, atol0, atol);
+    rep[j + 1][idx] = v;
+  }
+  if (c == 0) raised_T[s] = T + increment(T, 0, rtol, atol0, atol);
 }
+#endif
 
 // The number of systems still running (status 0), added to *count (cleared by the caller's launch of clear_count).
 extern "C" __global__ void __launch_bounds__(256) rosenbrock_running(unsigned n, const int* __restrict__ status, unsigned* __restrict__ count) {

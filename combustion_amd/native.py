@@ -78,6 +78,7 @@ SIGNATURES = {
     "cb_build_chemistry": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(_P)]),
     "cb_build_reactor": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "cb_build_jacobian": (C.c_int, [_P, C.c_int32, _I32P, C.c_int32, _I32P, C.POINTER(_P)]),
+    "cb_program_split": (C.c_int, [_P, C.c_int32, C.POINTER(_P), C.POINTER(_P)]),
     "cb_program_free": (None, [_P]),
     "cb_program_sizes": (C.c_int, [_P, _I32P, _I32P, _I64P, _I64P, _I32P]),
     "cb_program_op_counts": (C.c_int, [_P, _I64P]),
@@ -326,6 +327,13 @@ class Program:
         h = _P()
         _check(lib().cb_build_jacobian(self._h, len(wrt), wrt.ctypes.data_as(_I32P), len(rows), rows.ctypes.data_as(_I32P), C.byref(h)))
         return Program(h, self.mechanism)
+
+    def split(self, first_varying):
+        """cb_program_split: (shared, varying) programs; shared takes the inputs before first_varying and returns the
+        values the rest reads, varying takes this program's inputs followed by those values."""
+        s, v = _P(), _P()
+        _check(lib().cb_program_split(self._h, int(first_varying), C.byref(s), C.byref(v)))
+        return Program(s, self.mechanism), Program(v, self.mechanism)
 
     def hip_source(self, form="straight", block_size=0):
         return _text(lib().cb_program_hip_source, self._h, form_code(form), block_size)

@@ -111,6 +111,12 @@ int cb_build_reactor(const cb_mechanism* m, int32_t lean, cb_program** out);
  * d output[rows[i]] / d input[wrt[j]], row-major (n_rows x n_wrt). min and max differentiate the operand they select
  * (the first on ties), a select its chosen operand. Assemble it like any other program. */
 int cb_build_jacobian(const cb_program* p, int32_t n_wrt, const int32_t* wrt, int32_t n_rows, const int32_t* rows, cb_program** out);
+/* A program split at its dependence on the inputs from first_varying on (no counterpart in the reference; the batched
+ * integrator evaluates its difference-quotient copies this way): *shared computes, from the inputs before
+ * first_varying, the B values the rest of the program reads (its outputs); *varying takes p's inputs followed by
+ * those B values and computes p's outputs. shared once and varying per copy give p's results instruction for
+ * instruction, so for many states that differ only in the varying inputs the shared part is evaluated once. */
+int cb_program_split(const cb_program* p, int32_t first_varying, cb_program** shared, cb_program** varying);
 void cb_program_free(cb_program* p);
 int cb_program_sizes(const cb_program* p, int32_t* inputs, int32_t* outputs, int64_t* statements, int64_t* instructions, int32_t* registers);
 /* cuda.rs:229-254 categories over the linear program: alu, div, sqrt, exp, ln, neg, select, constants. */

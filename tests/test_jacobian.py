@@ -65,11 +65,14 @@ def central_differences(lin, X, wrt, h_rel=1e-6):
     return J
 
 
-def test_reactor_program_matches_the_oracle():
-    """cb_build_reactor: dT/dt of the energy balance and dY_k/dt = W_k ω_k / ρ against the oracle."""
-    M, mech, o = setup("h2o2-10")
-    T, P, Y = synth.config_states("h2o2-10", 64, 95)
-    got = oracle.interpret(M.reactor().linear(), np.vstack([T, P, Y]))
+@pytest.mark.parametrize("name,lean", [("h2o2-10", False), ("h2o2-10", True), ("gri-53", False), ("gri-53", True)])
+def test_reactor_program_matches_the_oracle(name, lean):
+    """cb_build_reactor: dT/dt of the energy balance and dY_k/dt = W_k ω_k / ρ against the oracle, in the reference's
+    shape and in the lean one (rate-of-progress products started from each reaction's rate constant) that the
+    integrator's kernels compile."""
+    M, mech, o = setup(name)
+    T, P, Y = synth.config_states(name, 64, 95)
+    got = oracle.interpret(M.reactor(lean=lean).linear(), np.vstack([T, P, Y]))
     for s in range(64):
         omega, dT, x = o.rhs_full(T[s], P[s], Y[:, s])
         rho = P[s] / (R * T[s] * np.sum(Y[:, s] / mech["W"]))
@@ -102,3 +105,29 @@ def test_reactor_jacobian_matches_central_differences(name, n):
     err = np.abs(analytic - fd) / np.maximum(np.abs(analytic), scale)
     print(name, "max error of the analytic Jacobian against central differences", err.max(), "instructions", J.n_instructions)
     assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["h2o2-10", "gri-53"])
+def test_split_at_the_mass_fractions_gives_the_program_results(name):
+    """cb_program_split of the reactor program at the mass fractions (inputs 2..K+1): the shared part from T and P,
+    then the varying part from the inputs and the shared values, reproduce the whole program's outputs bit for bit
+    in the interpreter, also for copies whose mass fractions differ from the state's (the integrator's difference
+    quotients); the shared part holds most of the work (the rate constants and Gibbs exponentials)."""
+    M, mech, o = setup(name)
+    T, P, Y = synth.config_states(name, 32, 41)
+    f = M.reactor(lean=True)
+    shared, varying = f.split(2)
+    assert shared.n_inputs == 2 and varying.n_inputs == 2 + M.K + shared.n_outputs
+    assert varying.n_outputs == f.n_outputs
+    X = np.vstack([T, P, Y])
+    B = oracle.interpret(shared.linear(), X[:2])
+    np.testing.assert_array_equal(oracle.interpret(varying.linear(), np.vstack([X, B])), oracle.interpret(f.linear(), X))
+    Yp = Y.copy()
+    Yp[3] *= 1.0 + 1e-8  # a perturbed copy: same T and P, so the same shared values
+    Xp = np.vstack([T, P, Yp])
+    np.testing.assert_array_equal(oracle.interpret(varying.linear(), np.vstack([Xp, B])), oracle.interpret(f.linear(), Xp))
+    whole, part = f.op_counts()["weighted"], varying.op_counts()["weighted"]
+    print(name, "shared values", shared.n_outputs, "weighted work of the varying part", part, "of", whole)
+    assert part < 0.4 * whole
+    with pytest.raises(Exception):
+        f.split(M.K + 3)
