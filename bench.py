@@ -151,7 +151,7 @@ def surrogate_workload(args, M, prog, dev):
                    "P ~ U[1,40] atm, fuel/O2/N2",
            "fp64": {"states_per_s": N * args.steps / wall_p, "kernel_ms": ms_p}}
     trained = sur.Trained.load(args.surrogate)
-    # the FP32-core rates beside the network (the faster pairing), and the MFMA mixed rates (the north star's)
+    # the FP32-core rates beside the network (the faster pairing), and the MFMA mixed rates (the mixed form's own)
     for rates in ("fast", "mixed"):
         smod = sur.assemble(prog, trained, rates=rates)
         wall_s, ms_s = time_module(smod, inputs, outputs, args.steps, args.warmup)

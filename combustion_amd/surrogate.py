@@ -110,7 +110,7 @@ def log_baseline(mechanism, T, P, Y, major, D=4, T0=1000.0):
     return out
 
 
-def judge(mechanism, T, P, Y, device="cuda", chunk=1 << 20):
+def exact_transport(mechanism, T, P, Y, device="cuda", chunk=1 << 20):
     """The plain FP64 transport outputs [K+2][n] (tiled FP64 form on the GPU)."""
     import torch
     prog = mechanism.chemistry(native.TRANSPORT)
@@ -139,7 +139,7 @@ def initial_params(net, rng):
 
 def train(mechanism, T, P, Y, props, width=256, depth=3, steps=4000, batch=16384, lr=2e-3, seed=0, device="cuda", log=None,
           hard=0.0, refresh=1000, common_scale=False, n_major=20):
-    """Fits the network to props [K+2][n] (the judge's outputs) on the GPU with the project's kernels.
+    """Fits the network to props [K+2][n] (the exact FP64 transport kernel's outputs) on the GPU with the project's kernels.
 
     hard > 0 mines hard examples: every `refresh` steps the network is evaluated on a random sample of the training
     states and a fraction `hard` of each later batch is drawn from the 5% of that sample with the largest error
@@ -250,7 +250,7 @@ def fallback_count(module, stream=None):
 
 
 def main(argv=None):
-    """Generates trajectory states, computes the judge's transport on them, trains the network and saves it:
+    """Generates trajectory states, computes the exact FP64 transport on them, trains the network and saves it:
     python -m combustion_amd.surrogate --mechanism gri-53 --out surrogate_gri53.npz"""
     import argparse
     import time
@@ -279,8 +279,8 @@ def main(argv=None):
     T, P, Y = reactor.trajectories(mech, a.trajectories, a.states, a.seed, t_end=a.t_end, max_steps=a.max_steps, log=log)
     log(f"{len(T)} training states from {a.trajectories} trajectories in {time.time() - t:.1f} s; T in [{T.min():.0f}, {T.max():.0f}]")
     t = time.time()
-    props = judge(mech, T, P, Y)
-    log(f"judge transport in {time.time() - t:.1f} s")
+    props = exact_transport(mech, T, P, Y)
+    log(f"exact transport in {time.time() - t:.1f} s")
     t = time.time()
     trained = train(mech, T, P, Y, props, width=a.width, depth=a.depth, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log,
                     hard=a.hard, common_scale=a.common_scale, n_major=a.major)

@@ -24,11 +24,11 @@ def evaluate(mod, T, P, Y, gpu):
 
 @pytest.fixture(scope="module")
 def small(gpu):
-    """A network trained briefly on 2^14 states of 64 trajectories (the states, their judge outputs, the network)."""
+    """A network trained briefly on 2^14 states of 64 trajectories (the states, their exact transport, the network)."""
     M, mech, o = setup("gri-53")
     T, P, Y = reactor.trajectories(M, 64, 1 << 14, seed=3, max_steps=200, batch=64)
     assert np.all(np.isfinite(T)) and T.min() > 200 and np.allclose(Y.sum(0), 1)
-    props = surrogate.judge(M, T, P, Y)
+    props = surrogate.exact_transport(M, T, P, Y)
     trained = surrogate.train(M, T, P, Y, props, steps=300, batch=4096)
     return T, P, Y, trained
 
@@ -120,7 +120,7 @@ def test_trained_network_meets_the_stated_bound(gpu):
 
 @pytest.mark.gpu
 def test_surrogate_with_the_mfma_mixed_rates(gpu):
-    """The north star's pairing: the network's transport beside the MFMA mixed-precision rates kernel (split-BF16
+    """The mixed form's pairing: the network's transport beside the MFMA mixed-precision rates kernel (split-BF16
     contractions, FP32 exp/log, FP64 ΔG/RT and forward-minus-reverse), on held-out trajectory states: ω within the
     mixed form's 1e-5 of the state floor and transport within 1e-3 of the FP64 oracle; ω the same bits as the mixed
     form's own."""
