@@ -113,3 +113,44 @@ def test_integrator_limits_are_reported(gpu):
     with pytest.raises(ValueError):
         integ.advance(x[:-1], x[0].clone(), x[0].clone(), x[0].clone(), torch.zeros(4, dtype=torch.int32, device=gpu),
                       torch.zeros(4, dtype=torch.int32, device=gpu), 1e-3)
+
+
+def integrate(gpu, M, y0, P, t_end, **flags):
+    """Reactors from y0 [K+1][n] at pressures P [n] to t_end: (y, steps) on the host."""
+    import torch
+    n = y0.shape[1]
+    y = torch.from_numpy(np.ascontiguousarray(y0)).to(gpu)
+    t = torch.zeros(n, dtype=torch.float64, device=gpu)
+    h = torch.full((n,), 1e-9, dtype=torch.float64, device=gpu)
+    status = torch.zeros(n, dtype=torch.int32, device=gpu)
+    steps = torch.zeros(n, dtype=torch.int32, device=gpu)
+    Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, **flags).advance(y, torch.from_numpy(np.ascontiguousarray(P)).to(gpu), t, h,
+                                                                         status, steps, t_end, max_steps=200000)
+    torch.cuda.synchronize()
+    assert torch.all(status == 1)
+    return y.cpu().numpy(), steps.cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mechanism,jacobian,t_end", [("handwritten", "analytic", 1e-3), ("handwritten", "differences", 1e-3),
+                                                      ("gri-53", "differences", 1e-5)])
+def test_a_reactor_does_not_depend_on_its_batch(gpu, mechanism, jacobian, t_end):
+    """Each reactor's steps are its own: integrated alone, or at another place in a batch of another size (another
+    lane group of a packed wave, another block and XCD), a reactor ends in the same state to the bit after the same
+    number of steps."""
+    from combustion_amd import synth
+    if mechanism == "handwritten":
+        d = np.load(os.path.join(GOLDEN, "reactor_handwritten.npz"))
+        M = Mechanism(os.path.join(GOLDEN, "handwritten.yaml"))
+    else:
+        d = np.load(os.path.join(GOLDEN, "reactor_gri53.npz"))
+        M = Mechanism(synth.mechanism_path(mechanism))
+    y0, P = np.vstack([d["T0"], d["Y0"]]), d["P"]
+    y, steps = integrate(gpu, M, y0, P, t_end, jacobian=jacobian)
+    one, one_steps = integrate(gpu, M, y0[:, 1:2], P[1:2], t_end, jacobian=jacobian)
+    np.testing.assert_array_equal(one[:, 0], y[:, 1])
+    assert one_steps[0] == steps[1]
+    order = np.arange(len(P))[::-1]
+    rev, rev_steps = integrate(gpu, M, np.tile(y0[:, order], 3), np.tile(P[order], 3), t_end, jacobian=jacobian)
+    np.testing.assert_array_equal(rev[:, :len(P)], y[:, order])
+    np.testing.assert_array_equal(rev_steps[:len(P)], steps[order])
