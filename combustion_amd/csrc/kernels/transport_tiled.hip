@@ -9,7 +9,7 @@
 //       FP32), species × states, one 16-row tile of k per wave at a time
 //   ρD_k = √T (Σ_j W_j x_j - W_k x_k) / (R S_k),  S_k = Σ_{j≠k} x_j / d_kj(ℓ)     (transport.rs:162-177)
 //       d_kj for j < k on MFMA: (16 pairs × 4 coefficients)·(ℓ⁰..ℓ³ × 16 states) per tile, a tile being 16
-//       consecutive j of one k; 1/d on VALU. Row by row (k = 1..K-1), the waves split row k's tiles by j block
+//       consecutive j of one k; 1/d on VALU (in FP64 four at a time from the reciprocal of their product). Row by row (k = 1..K-1), the waves split row k's tiles by j block
 //       (block b always to wave b mod WAVES): each adds x_k/d_kj into S_j in LDS (no other wave writes S_j) and sums
 //       x_j/d_kj across its lanes; after a barrier the owner of k's block adds the waves' row sums, in wave order,
 //       into S_k. One S array serves the block (it reuses the 1/v_k array of the viscosity stage), and every state
@@ -35,6 +35,17 @@ __device__ __forceinline__ real rcp(real b) {
   return __builtin_fma(r, e, r);
 }
 __device__ __forceinline__ real sqrt_real(double x) { return __builtin_sqrt(x); }
+// Four reciprocals from one: 1/d0 = d1 (d2 d3) / (d0 d1 d2 d3) and so on (a few ulps; the d are binary-diffusion
+// fits, far from the ends of the FP64 range)
+__device__ __forceinline__ void rcp4(const real (&d)[4], real (&r)[4]) {
+  const real p01 = d[0] * d[1], p23 = d[2] * d[3];
+  const real q = rcp(p01 * p23);
+  const real q01 = q * p23, q23 = q * p01;
+  r[0] = q01 * d[1];
+  r[1] = q01 * d[0];
+  r[2] = q23 * d[3];
+  r[3] = q23 * d[2];
+}
 #else
 #define ROW(g, i) (4 * (g) + (i))
 typedef float real;
@@ -42,6 +53,10 @@ typedef float r4 __attribute__((ext_vector_type(4)));
 #define MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0)
 __device__ __forceinline__ real rcp(real b) { return __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ real sqrt_real(double x) { return __builtin_sqrtf((float)x); }
+__device__ __forceinline__ void rcp4(const real (&d)[4], real (&r)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = rcp(d[i]);
+}
 #endif
 
 #define NS 16
@@ -188,13 +203,16 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
       r4 d = {0, 0, 0, 0};
       d = MFMA(dcoef[(size_t)t * (DK / 4) * 64 + lane], lp0, d);
       if (DK > 4) d = MFMA(dcoef[((size_t)t * (DK / 4) + 1) * 64 + lane], lp1, d);
+      real dv[4], R[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) dv[i] = j0 + ROW(grp, i) < k ? d[i] : (real)1;  // pairs past the diagonal: 1
+      rcp4(dv, R);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int j = j0 + ROW(grp, i);
         if (j < k) {
-          const real R = rcp(d[i]);
-          sk += L.x[j][col] * R;
-          L.S[j][col] += xk * R;
+          sk += L.x[j][col] * R[i];
+          L.S[j][col] += xk * R[i];
         }
       }
     }
