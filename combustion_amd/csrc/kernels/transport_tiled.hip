@@ -10,10 +10,11 @@
 //   ρD_k = √T (Σ_j W_j x_j - W_k x_k) / (R S_k),  S_k = Σ_{j≠k} x_j / d_kj(ℓ)     (transport.rs:162-177)
 //       d_kj for j < k on MFMA: (16 pairs × 4 coefficients)·(ℓ⁰..ℓ³ × 16 states) per tile, a tile being 16
 //       consecutive j of one k; 1/d on VALU (in FP64 four at a time from the reciprocal of their product). Row by row (k = 1..K-1), the waves split row k's tiles by j block
-//       (block b always to wave b mod WAVES): each adds x_k/d_kj into S_j in LDS (no other wave writes S_j) and sums
-//       x_j/d_kj across its lanes; after a barrier the owner of k's block adds the waves' row sums, in wave order,
-//       into S_k. One S array serves the block (it reuses the 1/v_k array of the viscosity stage), and every state
-//       adds the terms in the same order.
+//       (block b always to wave b mod WAVES): each adds x_k/d_kj into its register sums for S_j (no other wave
+//       touches S_j) and sums x_j/d_kj across its lanes; after a barrier the owner of k's block adds the waves' row
+//       sums, in wave order, into S_k in LDS, and after the last row each wave adds its register sums there. One S
+//       array serves the block (it reuses the 1/v_k array of the viscosity stage), and every state adds the terms in
+//       the same order.
 // Outputs λ [W/m/K], μ [Pa s], D_k = ρD_k/ρ [m²/s] with ρ = Σ W_k x_k.
 // Required defines: K, KP (K rounded up to 16), NTILE (diffusion tiles, K-1 rows of ⌈k/16⌉), DK (fit length
 // rounded up to 4: 4 or 8),
@@ -63,6 +64,7 @@ __device__ __forceinline__ void rcp4(const real (&d)[4], real (&r)[4]) {
 #define WAVES 4
 #define RGAS (1.380649e-23 * 6.02214076e23)
 #define KG ((K + 15) / 16)  // species per lane group (16 groups: 4 per wave)
+#define QB ((KP / 16 + WAVES - 1) / WAVES)  // diffusion column blocks per wave
 
 struct Lds {
   real x[KP][NS];  // concentrations (0 past K)
@@ -193,12 +195,21 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
   __syncthreads();
   const real lp0 = L.pw[grp][col];
   const real lp1 = DK > 4 ? L.pw[(4 + grp) % DK][col] : (real)0;
+  // the column sums S_j += x_k / d_kj of this wave's j blocks (jb = wave + WAVES q) stay in registers until the end
+  real acc[QB][4];
+#pragma unroll
+  for (int q = 0; q < QB; q++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[q][i] = 0;
   int t0 = 0;
   for (int k = 1; k < K; k++) {
     const int nb = (k + 15) / 16;
     const real xk = L.x[k][col];
     real sk = 0;
-    for (int jb = wave; jb < nb; jb += WAVES) {
+#pragma unroll
+    for (int q = 0; q < QB; q++) {
+      const int jb = wave + WAVES * q;
+      if (jb >= nb) break;  // uniform
       const int t = t0 + jb, j0 = 16 * jb;
       r4 d = {0, 0, 0, 0};
       d = MFMA(dcoef[(size_t)t * (DK / 4) * 64 + lane], lp0, d);
@@ -212,7 +223,7 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
         const int j = j0 + ROW(grp, i);
         if (j < k) {
           sk += L.x[j][col] * R[i];
-          L.S[j][col] += xk * R[i];
+          acc[q][i] += xk * R[i];
         }
       }
     }
@@ -230,6 +241,14 @@ transport_tiled(const double* const* __restrict__ in, double* const* __restrict_
     }
     t0 += nb;
   }
+  // each S_j has one owner wave: its column sums join the row sums already there
+#pragma unroll
+  for (int q = 0; q < QB; q++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int j = 16 * (wave + WAVES * q) + ROW(grp, i);
+      if (j < KP) L.S[j][col] += acc[q][i];
+    }
   const double mu_all = block_sum(L, 1, (double)mu, wave, grp, col);
 
   // 4. Outputs: λ and μ by one lane per state, D_k for this lane's species.
