@@ -18,10 +18,7 @@ extern const char* const rosenbrock_source;
 
 namespace {
 constexpr int kPollEvery = 8;
-}
 
-// The right-hand side program, its Jacobian program (d [dT/dt, dY/dt] / d [T, Y]; P is a parameter) and the
-// Rosenbrock kernels' source, for a mechanism.
 // One wave per block for the right-hand-side and Jacobian programs: a launch over n reactors then spreads over
 // every CU (2^14 reactors in blocks of 256 states would occupy 64 of the 256 CUs).
 constexpr unsigned kProgramBlock = 64;
@@ -69,7 +66,10 @@ ast::Function placeholder(const ast::Linear& l) {  // the Function of a program 
   f.output.assign(l.output.size(), -1);
   return f;
 }
+}  // namespace
 
+// The right-hand side program, its Jacobian program (d [dT/dt, dY/dt] / d [T, Y]; P is a parameter) or its split
+// for difference quotients, and the Rosenbrock kernels' source, for a mechanism.
 struct IntegratorPrograms {
   int m;
   bool differences;
@@ -194,9 +194,9 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   if (!differences_) tab_jac_ = table(rows(jac_, m + m * m));
   tab_F_ = table(rows(F_, m));
   if (differences_) {
-    // the shared values (B per reactor, reactor by reactor); 64 copies of every state (K + 2 input rows and M output
-    // rows of 64 n doubles, the input table followed by the shared array); the temperature-raised states (T row,
-    // then P and y's rows)
+    // the shared values (B per reactor, reactor by reactor); C copies of every state (C the power of two >= K + 1;
+    // K + 2 input rows and M output rows of C n doubles, the input table followed by the shared array); the
+    // temperature-raised states (T row, then P and y's rows)
     const size_t B = shared_linear_.output.size(), nr = n * copies_per_reactor(m_);
     double* shared = (double*)alloc(B * n * sizeof(double));
     double* rep_in = (double*)alloc((m + 1) * nr * sizeof(double));
@@ -249,8 +249,8 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
       jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
     }
     for (int phase = 1; phase <= 3; phase++) {
-      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &k1_, &k2_, &F1_, &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0,
-                      &atol};
+      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &k1_, &k2_, &F1_,
+                      &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
       device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,
                                           nullptr),
                     "launch rosenbrock");
