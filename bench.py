@@ -85,6 +85,21 @@ def cpu_baseline(prog, name, budget_s=8.0):
                       f"{oracle_rate:.0f} states/s on one core"}
 
 
+def issue_from_profiles(kernel, kernel_ms):
+    """VALU issue of a kernel against the SIMDs' issue capacity: its VALU instructions per launch (SQ_INSTS_VALU from the
+    committed counter profile) at 4 cycles per wave64 instruction on a 16-lane SIMD (16 for the quarter-rate FP64
+    transcendentals), over 1024 SIMDs at the 2.4 GHz peak clock for the kernel's time."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    cycles = c["waves_per_launch"] * (4 * c["valu_instructions_per_wave"] + 12 * c.get("trans_f64_instructions_per_wave", 0))
+    available = 1024 * 2.4e9 * kernel_ms * 1e-3
+    return {"valu_instructions_per_wave": c["valu_instructions_per_wave"], "waves": c["waves_per_launch"],
+            "issue_cycles": cycles, "frac": cycles / available}
+
+
 def traffic_from_profiles(kernel):
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if os.path.exists(path):
@@ -360,6 +375,7 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(dominant["kernel"]),
                      "kernel": dominant["kernel"], "flop_per_state": dominant["flop_per_state"],
                      "kernel_ms": dominant["kernel_ms"],
+                     "valu_issue": issue_from_profiles(dominant["kernel"], dominant["kernel_ms"]),
                      "step": {"kernel_ms": step_ms, "flop_per_state": flop_total,
                               "achieved": flop_total * N / (step_ms * 1e-3) / 1e12,
                               "frac": flop_total * N / (step_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
