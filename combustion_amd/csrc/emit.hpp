@@ -32,13 +32,11 @@ struct StraightOptions {
   std::vector<int32_t> lds_registers;
   unsigned fence_every = 128;
   // Inputs from `group_from` on (when >= 0) are shared by groups of `group_size` consecutive states (group
-  // id / group_size), each loaded where it is first used rather than at the top, so that they are not all live at
-  // once. Without `group_packed` input r of group g is in[r][g]; with it, the group's values are consecutive in one
-  // array, in[group_from][g · G + r - group_from] (G = the number of grouped inputs), and when the group size divides
-  // the block size a block stages its groups' values in LDS at once and reads them there.
+  // g = id / group_size; the group size divides the block size): the group's values are consecutive in one array,
+  // in[group_from][g · G + r - group_from] (G = the number of grouped inputs), which a block stages in LDS at once;
+  // each is read where first used rather than at the top, so that they are not all live at once.
   int32_t group_from = -1;
   unsigned group_size = 1;
-  bool group_packed = false;
   // Output o of state id is stored at out[o][id · output_stride] (with out[o] = base + o and the stride the number of
   // outputs, state by state).
   unsigned output_stride = 1;

@@ -74,8 +74,10 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
     << "(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n, unsigned offset) {\n"
     << "  const unsigned id = offset + blockIdx.x * blockDim.x + threadIdx.x;\n";
   const int32_t first_grouped = o.group_from >= 0 && o.group_from < l.n_input ? o.group_from : l.n_input;
-  const bool staged = first_grouped < l.n_input && o.group_packed && o.group_size > 0 && o.block_size % o.group_size == 0;
-  if (!staged) s << "  if (id >= n) return;\n";
+  const bool grouped = first_grouped < l.n_input;
+  if (grouped && (o.group_size == 0 || o.block_size % o.group_size != 0))
+    throw ast::Error("emit: the group size must divide the block size");
+  if (!grouped) s << "  if (id >= n) return;\n";
   // LDS-resident registers: slot of each, -1 for the others.
   std::vector<int32_t> slot(l.n_registers, -1);
   int32_t n_slots = 0;
@@ -91,32 +93,24 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   auto operand = [&](int32_t r) {
     return slot[r] >= 0 ? "lds_v[" + std::to_string(slot[r] * o.block_size) + " + tid]" : "r" + std::to_string(r);
   };
-  if (first_grouped < l.n_input) {
-    if (o.group_size == 0) throw ast::Error("emit: group size 0");
-    if (o.group_size == o.block_size)  // the block's group, uniform (the launch offset is a multiple of the block)
-      s << "  const unsigned gid = (offset + blockIdx.x * blockDim.x) / " << o.group_size << "u;\n";
+  if (grouped) {
+    // the block's groups staged in LDS by the whole block at once (one round trip to memory), then read where first
+    // used (the lanes of a group read one address: a broadcast); before the threads past n leave, and no further
+    // than the last group (every block holds at least one state; the launch offset is a multiple of the block)
+    const int32_t ng = l.n_input - first_grouped;
+    const unsigned per_block = o.block_size / o.group_size;
+    s << "  __shared__ double group_values[" << per_block * ng << "];\n"
+      << "  const unsigned first_group = (offset + blockIdx.x * blockDim.x) / " << o.group_size << "u;\n"
+      << "  const unsigned staged_values = min(" << per_block << "u, (n + " << o.group_size - 1 << "u) / " << o.group_size
+      << "u - first_group) * " << ng << "u;\n"
+      << "  const double* __restrict__ block_groups = in[" << first_grouped << "] + (size_t)first_group * " << ng << "u;\n"
+      << "  for (unsigned k = threadIdx.x; k < staged_values; k += blockDim.x) group_values[k] = block_groups[k];\n"
+      << "  __syncthreads();\n"
+      << "  if (id >= n) return;\n";
+    if (per_block == 1)  // the block's one group: uniform addresses
+      s << "  const double* grp = group_values;\n";
     else
-      s << "  const unsigned gid = id / " << o.group_size << "u;\n";
-    if (o.group_packed) {
-      const int32_t ng = l.n_input - first_grouped;
-      const unsigned per_block = o.block_size / o.group_size;
-      if (staged) {
-        // the block's groups staged in LDS by the whole block at once (one round trip to memory), then read where
-        // first used (the lanes of a group read one address: a broadcast); before the threads past n leave, and no
-        // further than the last group (every block holds at least one state)
-        s << "  __shared__ double group_values[" << per_block * ng << "];\n"
-          << "  const unsigned first_group = (offset + blockIdx.x * blockDim.x) / " << o.group_size << "u;\n"
-          << "  const unsigned staged_values = min(" << per_block << "u, (n + " << o.group_size - 1 << "u) / " << o.group_size
-          << "u - first_group) * " << ng << "u;\n"
-          << "  const double* __restrict__ block_groups = in[" << first_grouped << "] + (size_t)first_group * " << ng << "u;\n"
-          << "  for (unsigned k = threadIdx.x; k < staged_values; k += blockDim.x) group_values[k] = block_groups[k];\n"
-          << "  __syncthreads();\n"
-          << "  if (id >= n) return;\n"
-          << "  const double* grp = group_values + (gid - first_group) * " << ng << "u;\n";
-      } else {
-        s << "  const double* __restrict__ grp = in[" << first_grouped << "] + (size_t)gid * " << ng << "u;\n";
-      }
-    }
+      s << "  const double* grp = group_values + (id / " << o.group_size << "u - first_group) * " << ng << "u;\n";
   }
   const std::string out_index = o.output_stride == 1 ? "[id]" : "[(size_t)id * " + std::to_string(o.output_stride) + "u]";
   for (int i = 0; i < first_grouped; i++) s << "  const " << F << " r" << i << " = (" << F << ")in[" << i << "][id];\n";
@@ -124,10 +118,7 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   auto load_grouped = [&](int32_t r) {  // a grouped input, loaded before its first use
     if (r < first_grouped || r >= l.n_input || loaded[r]) return;
     loaded[r] = 1;
-    if (o.group_packed)
-      s << "  const " << F << " r" << r << " = (" << F << ")grp[" << r - first_grouped << "];\n";
-    else
-      s << "  const " << F << " r" << r << " = (" << F << ")in[" << r << "][gid];\n";
+    s << "  const " << F << " r" << r << " = (" << F << ")grp[" << r - first_grouped << "];\n";
   };
   std::vector<bool> is_bool(l.n_registers, false);
   // Outputs are stored as soon as their register is computed, ending their live range early.

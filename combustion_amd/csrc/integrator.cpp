@@ -56,7 +56,6 @@ StraightOptions varying_options(const ast::Linear& varying, int m) {
   StraightOptions o = rhs_options(varying);
   o.group_from = m + 1;  // after the inputs T, P, Y_0..Y_{K-1}
   o.group_size = copies_per_reactor(m);
-  o.group_packed = true;
   return o;
 }
 
