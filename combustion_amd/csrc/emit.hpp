@@ -31,12 +31,15 @@ struct StraightOptions {
   // kernels whose live values exceed the register file: LDS holds what would otherwise spill to scratch.
   std::vector<int32_t> lds_registers;
   unsigned fence_every = 128;
-  // Inputs from `group_from` on (when >= 0) are shared by groups of `group_size` consecutive states (group
-  // g = id / group_size; the group size divides the block size): the group's values are consecutive in one array,
-  // in[group_from][g · G + r - group_from] (G = the number of grouped inputs), which a block stages in LDS at once;
-  // each is read where first used rather than at the top, so that they are not all live at once.
-  int32_t group_from = -1;
-  unsigned group_size = 1;
+  // Group values: with group_table >= 0, groups of `group_size` consecutive states (group g = id / group_size; the
+  // group size divides the block size) share `group_stride` values, group g's at in[group_table][g · stride ..],
+  // which a block stages in LDS at once. Input r with group_offset[r] >= 0 is the group's value at that offset, read
+  // where first used rather than at the top (so that they are not all live at once), plus, for the state number
+  // increment_state[r] within its group (id mod group_size), the group's value at increment_offset[r] (>= 0); the
+  // other inputs are in[r][id]. Empty vectors: no grouped input.
+  int32_t group_table = -1;
+  unsigned group_size = 1, group_stride = 0;
+  std::vector<int32_t> group_offset, increment_offset, increment_state;
   // Output o of state id is stored at out[o][id · output_stride] (with out[o] = base + o and the stride the number of
   // outputs, state by state).
   unsigned output_stride = 1;

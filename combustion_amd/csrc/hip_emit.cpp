@@ -73,10 +73,15 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   s << name
     << "(const double* const* __restrict__ in, double* const* __restrict__ out, unsigned n, unsigned offset) {\n"
     << "  const unsigned id = offset + blockIdx.x * blockDim.x + threadIdx.x;\n";
-  const int32_t first_grouped = o.group_from >= 0 && o.group_from < l.n_input ? o.group_from : l.n_input;
-  const bool grouped = first_grouped < l.n_input;
-  if (grouped && (o.group_size == 0 || o.block_size % o.group_size != 0))
-    throw ast::Error("emit: the group size must divide the block size");
+  const bool grouped = o.group_table >= 0;
+  auto option = [&](const std::vector<int32_t>& v, int32_t r) { return r < (int32_t)v.size() ? v[r] : -1; };
+  if (grouped && (o.group_size == 0 || o.block_size % o.group_size != 0 || o.group_stride == 0))
+    throw ast::Error("emit: the group size must divide the block size, and groups need values");
+  for (int32_t r = 0; r < l.n_input; r++)
+    if (option(o.group_offset, r) >= (int32_t)o.group_stride || option(o.increment_offset, r) >= (int32_t)o.group_stride ||
+        ((option(o.group_offset, r) >= 0 || option(o.increment_offset, r) >= 0) && !grouped) ||
+        (option(o.increment_offset, r) >= 0 && option(o.group_offset, r) < 0))
+      throw ast::Error("emit: a group offset past the group's values, or without a group");
   if (!grouped) s << "  if (id >= n) return;\n";
   // LDS-resident registers: slot of each, -1 for the others.
   std::vector<int32_t> slot(l.n_registers, -1);
@@ -97,13 +102,13 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
     // the block's groups staged in LDS by the whole block at once (one round trip to memory), then read where first
     // used (the lanes of a group read one address: a broadcast); before the threads past n leave, and no further
     // than the last group (every block holds at least one state; the launch offset is a multiple of the block)
-    const int32_t ng = l.n_input - first_grouped;
+    const unsigned ng = o.group_stride;
     const unsigned per_block = o.block_size / o.group_size;
     s << "  __shared__ double group_values[" << per_block * ng << "];\n"
       << "  const unsigned first_group = (offset + blockIdx.x * blockDim.x) / " << o.group_size << "u;\n"
       << "  const unsigned staged_values = min(" << per_block << "u, (n + " << o.group_size - 1 << "u) / " << o.group_size
       << "u - first_group) * " << ng << "u;\n"
-      << "  const double* __restrict__ block_groups = in[" << first_grouped << "] + (size_t)first_group * " << ng << "u;\n"
+      << "  const double* __restrict__ block_groups = in[" << o.group_table << "] + (size_t)first_group * " << ng << "u;\n"
       << "  for (unsigned k = threadIdx.x; k < staged_values; k += blockDim.x) group_values[k] = block_groups[k];\n"
       << "  __syncthreads();\n"
       << "  if (id >= n) return;\n";
@@ -111,14 +116,24 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
       s << "  const double* grp = group_values;\n";
     else
       s << "  const double* grp = group_values + (id / " << o.group_size << "u - first_group) * " << ng << "u;\n";
+    bool increments = false;
+    for (int32_t r = 0; r < l.n_input; r++) increments |= option(o.increment_offset, r) >= 0;
+    if (increments) s << "  const unsigned member = id % " << o.group_size << "u;\n";
   }
   const std::string out_index = o.output_stride == 1 ? "[id]" : "[(size_t)id * " + std::to_string(o.output_stride) + "u]";
-  for (int i = 0; i < first_grouped; i++) s << "  const " << F << " r" << i << " = (" << F << ")in[" << i << "][id];\n";
+  for (int i = 0; i < l.n_input; i++)
+    if (option(o.group_offset, i) < 0) s << "  const " << F << " r" << i << " = (" << F << ")in[" << i << "][id];\n";
   std::vector<char> loaded(l.n_input, 0);
   auto load_grouped = [&](int32_t r) {  // a grouped input, loaded before its first use
-    if (r < first_grouped || r >= l.n_input || loaded[r]) return;
+    if (r >= l.n_input || option(o.group_offset, r) < 0 || loaded[r]) return;
     loaded[r] = 1;
-    s << "  const " << F << " r" << r << " = (" << F << ")grp[" << r - first_grouped << "];\n";
+    const int32_t inc = option(o.increment_offset, r);
+    s << "  const " << F << " r" << r << " = (" << F << ")";
+    if (inc >= 0)  // selected rather than adding zero, which would turn -0 into +0
+      s << "(member == " << option(o.increment_state, r) << "u ? grp[" << option(o.group_offset, r) << "] + grp[" << inc
+        << "] : grp[" << option(o.group_offset, r) << "]);\n";
+    else
+      s << "grp[" << option(o.group_offset, r) << "];\n";
   };
   std::vector<bool> is_bool(l.n_registers, false);
   // Outputs are stored as soon as their register is computed, ending their live range early.

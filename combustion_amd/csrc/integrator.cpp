@@ -1,7 +1,8 @@
 // Batched constant-pressure reactors on the GPU: per step, f and J (the analytic Jacobian program in one kernel, or
-// difference quotients: a replicate kernel, the right-hand side split at the mass fractions — its temperature and
-// pressure part once per reactor, the rest over the reactor's K+1 copies — and the whole right-hand side at the raised
-// temperature, whose outputs the first Rosenbrock phase turns into J), the three Rosenbrock phases of
+// difference quotients: a kernel writing each reactor's values and increments, the right-hand side split at the mass
+// fractions — its temperature and pressure part once per reactor, the rest over the reactor's K+1 copies, each copy
+// raising one component — and the whole right-hand side at the raised temperature, whose outputs the first
+// Rosenbrock phase turns into J), the three Rosenbrock phases of
 // kernels/rosenbrock.hip and two right-hand-side evaluations, all on the caller's stream; the batch is polled for
 // reactors still running every few steps.
 #include "integrator.hpp"
@@ -42,20 +43,35 @@ unsigned copies_per_reactor(int m) {
   return c;
 }
 
-// The part of the right-hand side shared by a reactor's copies (from T and P), its values stored reactor by reactor
-// (the B values of reactor s at [s B, (s + 1) B)).
-StraightOptions shared_options(const ast::Linear& shared) {
+// A reactor's values for its copies, reactor by reactor (stride B + 2M, M = K+1): the B values of the right-hand side's
+// shared part, then T, P, Y_0..Y_{K-1}, then the increments inc_1..inc_{M-1} of the components Y_0..Y_{K-1}.
+size_t group_stride(size_t B, int m) { return B + 2 * (size_t)m; }
+
+// The part of the right-hand side shared by a reactor's copies (from T and P), writing the first B of the reactor's
+// values.
+StraightOptions shared_options(const ast::Linear& shared, int m) {
   StraightOptions o = straight_options(1, kProgramBlock);
-  o.output_stride = (unsigned)shared.output.size();
+  o.output_stride = (unsigned)group_stride(shared.output.size(), m);
   return o;
 }
 
-// The part that varies with the mass fractions, over the copies: a block of 64 holds whole reactors, whose shared
-// values it stages in LDS.
+// The part that varies with the mass fractions, over the copies (a block of 64 holds whole reactors, whose values it
+// stages in LDS): every input from the reactor's values, copy j = 1..M-1 adding inc_j to component j (Y_{j-1}).
 StraightOptions varying_options(const ast::Linear& varying, int m) {
   StraightOptions o = rhs_options(varying);
-  o.group_from = m + 1;  // after the inputs T, P, Y_0..Y_{K-1}
+  const int32_t B = varying.n_input - (m + 1);
+  o.group_table = 0;
   o.group_size = copies_per_reactor(m);
+  o.group_stride = (unsigned)group_stride((size_t)B, m);
+  o.group_offset.resize(varying.n_input);
+  o.increment_offset.assign(varying.n_input, -1);
+  o.increment_state.assign(varying.n_input, -1);
+  for (int32_t r = 0; r <= m; r++) o.group_offset[r] = B + r;  // T, P, Y_0..Y_{K-1}
+  for (int32_t r = 2; r <= m; r++) {                           // Y_{r-2}, component r - 1
+    o.increment_offset[r] = B + m + r - 1;
+    o.increment_state[r] = r - 1;
+  }
+  for (int32_t b = 0; b < B; b++) o.group_offset[m + 1 + b] = b;
   return o;
 }
 
@@ -109,7 +125,7 @@ void compile_integrator(std::shared_ptr<const Mechanism> m, int flags) {
   compile_straight(ProgramView{m, &p.rhs_function, &p.rhs_linear, ProgramKind::Other, nullptr}, rhs_options(p.rhs_linear));
   if (!p.differences) compile_form(ProgramView{m, &p.jacobian_function, &p.jacobian_linear, ProgramKind::Other, nullptr}, 1, kProgramBlock);
   if (p.differences) {
-    compile_straight(ProgramView{m, &p.shared_function, &p.shared_linear, ProgramKind::Other, nullptr}, shared_options(p.shared_linear));
+    compile_straight(ProgramView{m, &p.shared_function, &p.shared_linear, ProgramKind::Other, nullptr}, shared_options(p.shared_linear, p.m));
     compile_straight(ProgramView{m, &p.varying_function, &p.varying_linear, ProgramKind::Other, nullptr},
                      varying_options(p.varying_linear, p.m));
   }
@@ -135,7 +151,7 @@ Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double a
     varying_function_ = std::move(p.varying_function);
     varying_linear_ = std::move(p.varying_linear);
     shared_ = assemble_straight(ProgramView{mech_, &shared_function_, &shared_linear_, ProgramKind::Other, nullptr},
-                                shared_options(shared_linear_));
+                                shared_options(shared_linear_, m_));
     varying_ = assemble_straight(ProgramView{mech_, &varying_function_, &varying_linear_, ProgramKind::Other, nullptr},
                                  varying_options(varying_linear_, m_));
   }
@@ -193,24 +209,20 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   if (!differences_) tab_jac_ = table(rows(jac_, m + m * m));
   tab_F_ = table(rows(F_, m));
   if (differences_) {
-    // the shared values (B per reactor, reactor by reactor); C copies of every state (C the power of two >= K + 1;
-    // K + 2 input rows and M output rows of C n doubles, the input table followed by the shared array); the
-    // temperature-raised states (T row, then P and y's rows)
+    // the reactors' values for their copies (group_stride per reactor); the right-hand sides of the C copies of
+    // every state (C the power of two >= K + 1; M rows of C n doubles); the temperature-raised states (T row, then P
+    // and y's rows)
     const size_t B = shared_linear_.output.size(), nr = n * copies_per_reactor(m_);
-    double* shared = (double*)alloc(B * n * sizeof(double));
-    double* rep_in = (double*)alloc((m + 1) * nr * sizeof(double));
+    groups_ = (double*)alloc(group_stride(B, m_) * n * sizeof(double));
     double* rep_out = (double*)alloc(m * nr * sizeof(double));
     double* t_out = (double*)alloc(m * n * sizeof(double));
     raised_T_ = (double*)alloc(n * sizeof(double));
-    std::vector<const void*> in_rows, out_rows;
-    for (size_t i = 0; i < m + 1; i++) in_rows.push_back(rep_in + i * nr);
-    in_rows.push_back(shared);
+    std::vector<const void*> out_rows, shared_out;
     for (size_t i = 0; i < m; i++) out_rows.push_back(rep_out + i * nr);
-    std::vector<const void*> shared_out;
-    for (size_t b = 0; b < B; b++) shared_out.push_back(shared + b);
+    for (size_t b = 0; b < B; b++) shared_out.push_back(groups_ + b);
     tab_shared_in_ = table({y, P});
     tab_shared_out_ = table(shared_out);
-    tab_rep_in_ = table(in_rows);
+    tab_rep_in_ = table({groups_});
     tab_rep_out_ = table(out_rows);
     std::vector<const void*> t_in = inputs(y);
     t_in[0] = raised_T_;
@@ -227,20 +239,22 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   if (n == 0) return 0;
   if (n >= (size_t(1) << 31)) throw ArgumentError("at most 2^31 - 1 reactors");
   prepare(y, P, n);
-  hipFunction_t running = nullptr, clear = nullptr, replicate = nullptr;
+  hipFunction_t running = nullptr, clear = nullptr, copies_kernel = nullptr;
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
-  if (differences_) device::check(hipModuleGetFunction(&replicate, step_.module, "rosenbrock_replicate"), "hipModuleGetFunction");
+  if (differences_) device::check(hipModuleGetFunction(&copies_kernel, step_.module, "rosenbrock_copies"), "hipModuleGetFunction");
   unsigned n32 = (unsigned)n;
   const unsigned systems_per_wave = systems_per_wave_;
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
   const size_t copies = n * (size_t)copies_per_reactor(m_);
+  unsigned stride = (unsigned)group_stride(shared_linear_.output.size(), m_), shared_values = (unsigned)shared_linear_.output.size();
   long taken = 0;
   for (int step = 0; step < max_steps; step++) {
     if (differences_) {
-      void* rargs[] = {&n32, &y, &P, &tab_rep_in_, &raised_T_, &rtol, &atol0, &atol};
-      device::check(hipModuleLaunchKernel(replicate, (unsigned)((copies + 255) / 256), 1, 1, 256, 1, 1, 0, stream, rargs, nullptr),
-                    "launch rosenbrock_replicate");
+      void* rargs[] = {&n32, &y, &P, &groups_, &stride, &shared_values, &raised_T_, &rtol, &atol0, &atol};
+      const size_t fields = n * 2 * (size_t)m_;
+      device::check(hipModuleLaunchKernel(copies_kernel, (unsigned)((fields + 255) / 256), 1, 1, 256, 1, 1, 0, stream, rargs, nullptr),
+                    "launch rosenbrock_copies");
       shared_->launch_only((const double* const*)tab_shared_in_, (double* const*)tab_shared_out_, n, stream, nullptr);
       varying_->launch_only((const double* const*)tab_rep_in_, (double* const*)tab_rep_out_, copies, stream, nullptr);
       rhs_->launch_only((const double* const*)tab_t_in_, (double* const*)tab_t_out_, n, stream, nullptr);

@@ -55,11 +55,11 @@ class Integrator {
   int* perm_ = nullptr;
   unsigned* running_ = nullptr;
   void **tab_y_ = nullptr, **tab_ys_ = nullptr, **tab_jac_ = nullptr, **tab_F_ = nullptr;
-  // difference quotients: the shared part's inputs [T, P] and outputs; the copies' inputs (followed by the shared
-  // values) and outputs; the temperature-raised states' inputs and outputs; their raised temperatures
+  // difference quotients: the shared part's inputs [T, P] and outputs; the copies' inputs (the reactors' values)
+  // and outputs; the temperature-raised states' inputs and outputs
   void **tab_shared_in_ = nullptr, **tab_shared_out_ = nullptr, **tab_rep_in_ = nullptr, **tab_rep_out_ = nullptr;
   void **tab_t_in_ = nullptr, **tab_t_out_ = nullptr;
-  double* raised_T_ = nullptr;
+  double *groups_ = nullptr, *raised_T_ = nullptr;  // each reactor's values for its copies; the raised temperatures
   const double* last_y_ = nullptr;
   const double* last_P_ = nullptr;
   void release();

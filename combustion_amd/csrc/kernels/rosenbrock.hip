@@ -19,11 +19,11 @@
 // rows M..M+M²-1 (the Jacobian program's outputs); lu[s][j][i] (column-major by system) then the M reciprocal pivots
 // of system s at lu[n M² + s M + i]; perm[s][k] = the row chosen at step k.
 // With JAC_DIFFERENCES defined (and COPIES, a power of two >= M), J is formed in phase 1 from difference quotients
-// instead: rosenbrock_replicate writes COPIES copies of every state, system by system (copy 0 and copies past M-1 the
-// state, copy j = 1..M-1 with component j raised by inc_j), and the state's temperature raised by inc_0; phase 1
-// reads the right-hand sides of the copies, rep_f[i][COPIES s + c] (row i of system s is COPIES consecutive
-// doubles), and at the raised temperature, rep_t[i][s]: J_ij = (f_i(copy j) - f_i(y)) / inc_j,
-// J_i0 = (f_i(raised T) - f_i(y)) / inc_0; f(y) goes to jac rows 0..M-1.
+// instead: the right-hand side is evaluated for COPIES copies of every state, system by system (copy 0 and copies past
+// M-1 the state, copy j = 1..M-1 with component j raised by inc_j, from the values rosenbrock_copies writes), and at
+// the state's temperature raised by inc_0; phase 1 reads the right-hand sides of the copies, rep_f[i][COPIES s + c]
+// (row i of system s is COPIES consecutive doubles), and at the raised temperature, rep_t[i][s]:
+// J_ij = (f_i(copy j) - f_i(y)) / inc_j, J_i0 = (f_i(raised T) - f_i(y)) / inc_0; f(y) goes to jac rows 0..M-1.
 // Required define: M; P (systems per wave) defaults to ⌊64/M⌋. Launch: ⌈n / P⌉ blocks of 64 threads.
 
 #define D_ROS 0.29289321881345248   // 1 / (2 + √2)
@@ -292,25 +292,29 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
 }
 
 #ifdef JAC_DIFFERENCES
-// Copies c = 0..COPIES-1 of every system's state as inputs [T, P, Y_0..Y_{M-2}] of the mass-fraction part of the
-// right-hand side over COPIES n states (copy c of system s at COPIES s + c): copy j = 1..M-1 with component j (Y_{j-1})
-// increased by inc_j, the others unchanged; and raised_T[s] = T + inc_0 for the whole right-hand side at the raised
-// temperature.
+// Each system's values for its copies, after the `shared` values of the right-hand side's part shared by the copies:
+// groups[s stride + shared + f] for f = 0..2M-1 is T, P, Y_0..Y_{M-2} (f < M + 1), then inc_j of component j = f - M
+// (j = 1..M-1); and raised_T[s] = T + inc_0 for the whole right-hand side at the raised temperature. One thread per
+// (system, f).
 extern "C" __global__ void __launch_bounds__(256)
-rosenbrock_replicate(unsigned n, const double* __restrict__ y, const double* __restrict__ P_in, double* const* __restrict__ rep,
-                     double* __restrict__ raised_T, double rtol, double atol0, double atol) {
+rosenbrock_copies(unsigned n, const double* __restrict__ y, const double* __restrict__ P_in, double* __restrict__ groups,
+                  unsigned stride, unsigned shared, double* __restrict__ raised_T, double rtol, double atol0, double atol) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n * COPIES) return;
-  const unsigned s = (unsigned)(idx / COPIES), c = (unsigned)(idx % COPIES);
-  const double T = y[s];
-  rep[0][idx] = T;
-  rep[1][idx] = P_in[s];
-  for (int j = 1; j < M; j++) {
-    double v = y[(size_t)j * n + s];
-    if ((int)c == j) v += increment(v, j, rtol, atol0, atol);
-    rep[j + 1][idx] = v;
+  if (idx >= (size_t)n * 2 * M) return;
+  const unsigned s = (unsigned)(idx / (2 * M)), f = (unsigned)(idx - (size_t)s * (2 * M));
+  double v;
+  if (f == 0) {
+    v = y[s];
+    raised_T[s] = v + increment(v, 0, rtol, atol0, atol);
+  } else if (f == 1) {
+    v = P_in[s];
+  } else if (f <= M) {
+    v = y[(size_t)(f - 1) * n + s];
+  } else {
+    const int j = (int)f - M;
+    v = increment(y[(size_t)j * n + s], j, rtol, atol0, atol);
   }
-  if (c == 0) raised_T[s] = T + increment(T, 0, rtol, atol0, atol);
+  groups[(size_t)s * stride + shared + f] = v;
 }
 #endif
 
