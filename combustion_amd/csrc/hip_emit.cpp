@@ -80,8 +80,9 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   for (int32_t r = 0; r < l.n_input; r++)
     if (option(o.group_offset, r) >= (int32_t)o.group_stride || option(o.increment_offset, r) >= (int32_t)o.group_stride ||
         ((option(o.group_offset, r) >= 0 || option(o.increment_offset, r) >= 0) && !grouped) ||
-        (option(o.increment_offset, r) >= 0 && option(o.group_offset, r) < 0))
-      throw ast::Error("emit: a group offset past the group's values, or without a group");
+        (option(o.increment_offset, r) >= 0 &&
+         (option(o.group_offset, r) < 0 || option(o.increment_state, r) < 0 || option(o.increment_state, r) >= (int32_t)o.group_size)))
+      throw ast::Error("emit: a group offset past the group's values, without a group, or an increment for no member");
   if (!grouped) s << "  if (id >= n) return;\n";
   // LDS-resident registers: slot of each, -1 for the others.
   std::vector<int32_t> slot(l.n_registers, -1);
