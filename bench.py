@@ -219,17 +219,17 @@ def large_mechanism_workload(args, dev):
     return out
 
 
-def integrator_workload(args, dev, jacobian="automatic", mechanism="handwritten"):
-    """The batched reactor integrator (cb_integrator_*, Rosenbrock ode23s at rtol 1e-8; the Jacobian analytic or by
-    difference quotients, "automatic" choosing by the state size) on constant-pressure reactors:
+def integrator_workload(args, dev, jacobian="automatic", mechanism="handwritten", method="rodas4"):
+    """The batched reactor integrator (cb_integrator_*, Rosenbrock ode23s or RODAS4 at rtol 1e-8; the Jacobian
+    analytic or by difference quotients, "automatic" choosing by the state size) on constant-pressure reactors:
     - "handwritten": 2^16 reactors of the hand-written H2/O2 mechanism (K=6; tests/golden/handwritten.yaml, the
       twelve initial states of tests/golden/reactor_handwritten.npz repeated with T0 scaled by U[0.95, 1.05]),
       through ignition to 1 ms;
     - "gri-53": 2^14 reactors of the 53-species mechanism (the four fuel/O2/N2 mixtures of
       tests/golden/reactor_gri53.npz repeated with T0 scaled by U[0.98, 1.02]), to 0.1 ms.
-    Work per reactor step: two right-hand-side evaluations, the Jacobian (the analytic program with its own
-    right-hand side, or difference quotients from the right-hand side split at the mass fractions), the LU of the
-    (K+1)² matrix and three solves."""
+    Work per reactor step: the Jacobian (the analytic program with its own right-hand side, or difference quotients
+    from the right-hand side split at the mass fractions), the LU of the (K+1)² matrix, and two right-hand-side
+    evaluations and three solves (ode23s) or five and six (RODAS4)."""
     import torch
     from combustion_amd import Mechanism, synth
     from combustion_amd.native import Integrator
@@ -249,7 +249,7 @@ def integrator_workload(args, dev, jacobian="automatic", mechanism="handwritten"
     T0 = np.tile(d["T0"], reps)[:n] * rng.uniform(1 - spread, 1 + spread, n)
     Y0 = np.tile(d["Y0"], (1, reps))[:, :n]
     P = np.tile(d["P"], reps)[:n]
-    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, jacobian=jacobian)
+    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, jacobian=jacobian, method=method)
 
     def run(t_end):
         y = torch.from_numpy(np.vstack([T0, Y0])).to(dev).contiguous()
@@ -277,10 +277,11 @@ def integrator_workload(args, dev, jacobian="automatic", mechanism="handwritten"
                     + 2 * m * m)
     else:
         jac_flop = rhs.jacobian([0] + [2 + k for k in range(M.K)]).op_counts()["weighted"]
-    flop_step = 2 * rhs.op_counts()["weighted"] + jac_flop + (2 * m ** 3) // 3 + 3 * 2 * m * m
+    stages = 3 if method == "ode23s" else 6
+    flop_step = (stages - 1) * rhs.op_counts()["weighted"] + jac_flop + (2 * m ** 3) // 3 + stages * 2 * m * m
     reactor_steps = int(steps.sum().item())
     rise = (y[0] - torch.from_numpy(T0).to(dev)).cpu().numpy()
-    return {"workload": f"{n} {what}, rtol 1e-8", "jacobian": used, "reactors": n, "batch_steps": taken, "wall_ms": wall * 1e3,
+    return {"workload": f"{n} {what}, rtol 1e-8", "method": method, "jacobian": used, "reactors": n, "batch_steps": taken, "wall_ms": wall * 1e3,
             "reactor_steps": reactor_steps, "reactor_steps_per_s": reactor_steps / wall,
             "steps_per_reactor": {"min": int(steps.min().item()), "max": int(steps.max().item())},
             "reached_t_end": float((status == 1).float().mean().item()),
@@ -307,6 +308,7 @@ def main():
                     help="for --only integrator")
     ap.add_argument("--integrator-mechanism", default="handwritten", choices=["handwritten", "gri-53"],
                     help="for --only integrator")
+    ap.add_argument("--integrator-method", default="rodas4", choices=["rodas4", "ode23s"], help="for --only integrator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every state is independent, so N GPUs would each run "
@@ -330,7 +332,7 @@ def main():
     outputs = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(prog.n_outputs)]
     if args.only == "integrator":
         print(json.dumps({"only": "integrator", **integrator_workload(args, dev, args.integrator_jacobian,
-                                                                     args.integrator_mechanism)}))
+                                                                     args.integrator_mechanism, args.integrator_method)}))
         return
     if args.only == "surrogate":
         from combustion_amd import reactor

@@ -2,8 +2,8 @@
 // difference quotients: a kernel writing each reactor's values and increments, the right-hand side split at the mass
 // fractions — its temperature and pressure part once per reactor, the rest over the reactor's K+1 copies, each copy
 // raising one component — and the whole right-hand side at the raised temperature, whose outputs the first
-// Rosenbrock phase turns into J), the three Rosenbrock phases of
-// kernels/rosenbrock.hip and two right-hand-side evaluations, all on the caller's stream; the batch is polled for
+// Rosenbrock phase turns into J), the Rosenbrock phases of kernels/rosenbrock.hip (three for ode23s, six for
+// RODAS4) with a right-hand-side evaluation between two phases, all on the caller's stream; the batch is polled for
 // reactors still running every few steps.
 #include "integrator.hpp"
 
@@ -94,7 +94,8 @@ struct IntegratorPrograms {
   IntegratorPrograms(const Mechanism& mech, int flags) : m((int)mech.species.len() + 1) {
     if (m > 64) throw Unsupported("the batched integrator holds a reactor's state in one wave: at most 63 species");
     const int jac = flags & kJacobianMask;
-    if ((flags & ~(kJacobianMask | kOneSystemPerWave)) || jac > kJacobianDifferences) throw ArgumentError("unknown integrator flags");
+    if ((flags & ~(kJacobianMask | kOneSystemPerWave | kMethodOde23s)) || jac > kJacobianDifferences)
+      throw ArgumentError("unknown integrator flags");
     differences = jac == kJacobianDifferences || (jac == kJacobianAutomatic && m > kAnalyticMaxState);
     rhs_function = reactor(mech, true);
     rhs_linear = ast::linearize(rhs_function);
@@ -114,6 +115,7 @@ struct IntegratorPrograms {
     std::ostringstream src;
     src << "#define M " << m << "\n";
     if (flags & kOneSystemPerWave) src << "#define P 1\n";
+    if (!(flags & kMethodOde23s)) src << "#define RODAS 1\n";
     if (differences) src << "#define JAC_DIFFERENCES 1\n#define COPIES " << copies_per_reactor(m) << "\n";
     src << rosenbrock_source;
     step_source = src.str();
@@ -137,6 +139,7 @@ Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double a
   if (!(rtol > 0) || !(atol_T > 0) || !(atol_Y > 0)) throw ArgumentError("tolerances must be positive");
   IntegratorPrograms p(*mech_, flags);
   differences_ = p.differences;
+  phases_ = (flags & kMethodOde23s) ? 3 : 6;
   systems_per_wave_ = (flags & kOneSystemPerWave) ? 1u : 64u / (unsigned)m_;
   rhs_function_ = std::move(p.rhs_function);
   rhs_linear_ = std::move(p.rhs_linear);
@@ -182,9 +185,7 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   jac_ = (double*)alloc((differences_ ? m : m + m * m) * n * sizeof(double));  // f(y), then J unless by differences
   F_ = (double*)alloc(m * n * sizeof(double));
   ys_ = (double*)alloc(m * n * sizeof(double));
-  k1_ = (double*)alloc(m * n * sizeof(double));
-  k2_ = (double*)alloc(m * n * sizeof(double));
-  F1_ = (double*)alloc(m * n * sizeof(double));
+  kst_ = (double*)alloc((phases_ == 6 ? 5 : 3) * m * n * sizeof(double));  // RODAS4: k1..k5; ode23s: k1, k2, F1
   lu_ = (double*)alloc((m * m + m) * n * sizeof(double));  // the factors, then the reciprocal pivots
   perm_ = (int*)alloc(m * n * sizeof(int));
   running_ = (unsigned*)alloc(256);
@@ -261,13 +262,13 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
     } else {
       jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
     }
-    for (int phase = 1; phase <= 3; phase++) {
-      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &k1_, &k2_, &F1_,
+    for (int phase = 1; phase <= phases_; phase++) {
+      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &kst_,
                       &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
       device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,
                                           nullptr),
                     "launch rosenbrock");
-      if (phase < 3) rhs_->launch_only((const double* const*)tab_ys_, (double* const*)tab_F_, n, stream, nullptr);
+      if (phase < phases_) rhs_->launch_only((const double* const*)tab_ys_, (double* const*)tab_F_, n, stream, nullptr);
     }
     taken++;
     if ((step + 1) % kPollEvery == 0 || step + 1 == max_steps) {

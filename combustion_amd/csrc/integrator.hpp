@@ -14,12 +14,13 @@
 
 namespace combustion {
 
-// Options (cb_integrator_create's flags): the Jacobian, and the Rosenbrock kernel's packing.
+// Options (cb_integrator_create's flags): the Jacobian, the Rosenbrock kernel's packing, and the method.
 constexpr int kJacobianAutomatic = 0;    // the analytic program up to kAnalyticMaxState rows, difference quotients above
 constexpr int kJacobianAnalytic = 1;     // the forward-mode program of cb_build_jacobian
 constexpr int kJacobianDifferences = 2;  // difference quotients from n (K+1) copies of the states (split right-hand side)
 constexpr int kJacobianMask = 0xF;
 constexpr int kOneSystemPerWave = 0x10;  // one reactor per Rosenbrock wave (readlane broadcasts) whatever its size
+constexpr int kMethodOde23s = 0x20;      // ode23s's (2,3) pair (three phases) instead of RODAS4 (six)
 constexpr int kAnalyticMaxState = 8;  // beyond, the analytic program spills (3.2 KB per lane at 11 rows) and loses to differences
 
 // Fills the code-object cache with the integrator's kernels for a mechanism (no GPU needed).
@@ -44,6 +45,7 @@ class Integrator {
   int m_;
   double rtol_, atol_T_, atol_Y_;
   bool differences_;
+  int phases_;  // Rosenbrock phases per step: 3 (ode23s) or 6 (RODAS4)
   unsigned systems_per_wave_;
   ast::Function rhs_function_, jacobian_function_, shared_function_, varying_function_;
   ast::Linear rhs_linear_, jacobian_linear_, shared_linear_, varying_linear_;
@@ -51,7 +53,7 @@ class Integrator {
   device::Kernel step_;
   size_t n_ = 0;
   std::vector<void*> buffers_;
-  double *jac_ = nullptr, *F_ = nullptr, *ys_ = nullptr, *k1_ = nullptr, *k2_ = nullptr, *F1_ = nullptr, *lu_ = nullptr;
+  double *jac_ = nullptr, *F_ = nullptr, *ys_ = nullptr, *kst_ = nullptr, *lu_ = nullptr;  // kst_: the stage vectors
   int* perm_ = nullptr;
   unsigned* running_ = nullptr;
   void **tab_y_ = nullptr, **tab_ys_ = nullptr, **tab_jac_ = nullptr, **tab_F_ = nullptr;

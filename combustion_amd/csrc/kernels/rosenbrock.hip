@@ -1,29 +1,39 @@
-// One step of the L-stable (2,3) Rosenbrock pair of Shampine & Reichelt ("The MATLAB ODE Suite", SIAM J. Sci.
-// Comput. 18, 1997; the method of ode23s) for a batch of independent systems y' = f(y) of size M <= 64, in three
-// phases between evaluations of f (and of f with its Jacobian J) by other kernels:
-//   phase 1 (after [f(y), J(y)]):  W = I - h d J factored (LU, partial pivoting); k1 = W⁻¹ f(y); ys = y + h k1 / 2
+// One step of a Rosenbrock method for a batch of independent systems y' = f(y) of size M <= 64, in phases between
+// evaluations of f (and of f with its Jacobian J) by other kernels. Two methods, W = I - h γ J factored once per step:
+// the L-stable (2,3) pair of Shampine & Reichelt ("The MATLAB ODE Suite", SIAM J. Sci. Comput. 18, 1997; the method
+// of ode23s), γ = d, in three phases:
+//   phase 1 (after [f(y), J(y)]):  W factored (LU, partial pivoting); k1 = W⁻¹ f(y); ys = y + h k1 / 2
 //   phase 2 (after F1 = f(ys)):    k2 = W⁻¹ (F1 - k1) + k1; ys = y + h k2
-//   phase 3 (after F2 = f(ys)):    k3 = W⁻¹ (F2 - e32 (k2 - F1) - 2 (k1 - f(y))); error = h (k1 - 2 k2 + k3) / 6;
-//                                   accept (y = ys, t += h) when max_i |error_i| / (atol_i + rtol max(|y_i|, |ys_i|)) <= 1,
-//                                   and the next h from that ratio, clipped to the end time
-// with d = 1/(2+√2) and e32 = 6+√2. A wave holds P = ⌊64/M⌋ systems, M lanes each: lane M p + i holds row i of
+//   phase 3 (after F2 = f(ys)):    k3 = W⁻¹ (F2 - e32 (k2 - F1) - 2 (k1 - f(y))); error = h (k1 - 2 k2 + k3) / 6,
+//                                   y_new = ys
+// with d = 1/(2+√2) and e32 = 6+√2; or, with RODAS defined, the stiffly accurate order-4 method RODAS4 of Hairer &
+// Wanner with its embedded order-3 solution (γ = 1/4), in six phases:
+//   phase 1 (after [f(y), J(y)]):  W factored; k1 = W⁻¹ γ h f(y); ys = Y_2 = y + a_21 k1
+//   phase I = 2..6 (after f(Y_I)): k_I = W⁻¹ γ (h f(Y_I) + Σ_{j<I} c_Ij k_j); ys = Y_{I+1} = y + Σ_{j<=I} a_{I+1,j} k_j
+//                                   (I < 6); error = k6, y_new = Y_6 + k6
+// The last phase accepts (y = y_new, t += h) when max_i |error_i| / (atol_i + rtol max(|y_i|, |y_new,i|)) <= 1 and
+// takes the next h from that ratio (0.9 ratio^(-1/q), q the order of the error estimate's local error, within
+// [0.2, 5] for ode23s and [0.2, 6] for RODAS4, at most 0.5 after a rejection), clipped to the end time.
+// A wave holds P = ⌊64/M⌋ systems, M lanes each: lane M p + i holds row i of
 // system p's W (M doubles in registers). With one system per wave the pivot row of each elimination step is
 // wave-uniform and broadcast with readlane; with several, every lane reads its system's pivot row with a cross-lane
 // permute (ds_bpermute), the pivot search and the error norm loop over the system's M lanes, and lanes whose system
 // is finished or past the batch compute along without storing. The LU factors and pivot order are kept in global
-// memory for phases 2 and 3, with the reciprocal of each row's pivot, so that the solves multiply instead of divide.
+// memory for the later phases, with the reciprocal of each row's pivot, so that the solves multiply instead of divide.
 // Systems whose status is not 0 (running) are left alone. Blocks take their systems in XCD-aware order: the 8 XCDs
 // receive blocks in turn, and block b works on system group xcd·⌊B/8⌋ + min(xcd, B mod 8) + ⌊b/8⌋ (B blocks), so
 // the systems that share a cache line of the SoA vectors run under one L2.
 // Layouts: vectors are SoA, component i of system s at [i * n + s]; jac holds f(y) in rows 0..M-1 and J row-major in
 // rows M..M+M²-1 (the Jacobian program's outputs); lu[s][j][i] (column-major by system) then the M reciprocal pivots
-// of system s at lu[n M² + s M + i]; perm[s][k] = the row chosen at step k.
+// of system s at lu[n M² + s M + i]; perm[s][k] = the row chosen at step k; kst holds the stage vectors, M n doubles
+// each (ode23s: k1, k2, F1; RODAS4: k1..k5).
 // With JAC_DIFFERENCES defined (and COPIES, a power of two >= M), J is formed in phase 1 from difference quotients
 // instead: the right-hand side is evaluated for COPIES copies of every state, system by system (copy 0 and copies past
 // M-1 the state, copy j = 1..M-1 with component j raised by inc_j, from the values rosenbrock_copies writes), and at
 // the state's temperature raised by inc_0; phase 1 reads the right-hand sides of the copies, rep_f[i][COPIES s + c]
 // (row i of system s is COPIES consecutive doubles), and at the raised temperature, rep_t[i][s]:
-// J_ij = (f_i(copy j) - f_i(y)) / inc_j, J_i0 = (f_i(raised T) - f_i(y)) / inc_0; f(y) goes to jac rows 0..M-1.
+// J_ij = (f_i(copy j) - f_i(y)) / inc_j, J_i0 = (f_i(raised T) - f_i(y)) / inc_0; for ode23s's phase 3, f(y) goes to jac
+// rows 0..M-1.
 // Required define: M; P (systems per wave) defaults to ⌊64/M⌋. Launch: ⌈n / P⌉ blocks of 64 threads.
 
 #define D_ROS 0.29289321881345248   // 1 / (2 + √2)
@@ -109,11 +119,63 @@ __device__ __forceinline__ double system_max(double v, int base) {
 #endif
 }
 
+#ifdef RODAS
+// The stage coefficients of RODAS4 in its transformed form (Hairer & Wanner, "Solving Ordinary Differential
+// Equations II", 2nd ed., Springer 1996, section VI.4, and their code RODAS): a_ij of the stage points, c_ij of the
+// stage right-hand sides (stage 6's point is stage 5's combination plus k5: a_6j = a_5j for j < 5, a_65 = 1).
+__device__ __forceinline__ constexpr double rodas_a(int i, int j) {
+  return i == 2 ? 1.544
+       : i == 3 ? (j == 1 ? 0.9466785280815826 : 0.2557011698983284)
+       : i == 4 ? (j == 1 ? 3.314825187068521 : j == 2 ? 2.896124015972201 : 0.9986419139977817)
+       : j == 1 ? 1.221224509226641 : j == 2 ? 6.019134481288629 : j == 3 ? 12.53708332932087 : j == 4 ? -0.6878860361058950 : 1.0;
+}
+__device__ __forceinline__ constexpr double rodas_c(int i, int j) {
+  return i == 2 ? -5.6688
+       : i == 3 ? (j == 1 ? -2.430093356833875 : -0.2063599157091915)
+       : i == 4 ? (j == 1 ? -0.1073529058151375 : j == 2 ? -9.594562251023355 : -20.47028614809616)
+       : i == 5 ? (j == 1 ? 7.496443313967647 : j == 2 ? -10.24680431464352 : j == 3 ? -33.99990352819905 : 11.70890893206160)
+       : j == 1 ? 8.083246795921522 : j == 2 ? -7.981132988064893 : j == 3 ? -31.52159432874371 : j == 4 ? 16.31930543123136
+                : -6.058818238834054;
+}
+#define GAMMA 0.25
+#define LAST_PHASE 6
+#define ERROR_ORDER 4.0  // the embedded estimate's local error is O(h^4)
+#define FACTOR_MAX 6.0
+
+// Stage I = 2..6 from f(Y_I) in F and the stages before it: k_I = W⁻¹ γ (h f(Y_I) + Σ_{j<I} c_Ij k_j); for I < 6
+// stores k_I and the next stage point Y_{I+1} = y + Σ_{j<=I} a_{I+1,j} k_j in ys; returns k_I.
+template <int I>
+__device__ __forceinline__ double rodas_stage(const double (&a)[M], int pos, const int (&perm)[M], double rd, int i, int base, bool row,
+                                              bool active, unsigned n, unsigned s, unsigned sv, const double* __restrict__ F,
+                                              const double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst, double hs) {
+  const int ir = row ? i : 0;
+  const size_t mn = (size_t)M * n;
+  double r = row ? hs * F[(size_t)ir * n + sv] : 0.0;
+  double z = I < 6 && row ? y[(size_t)ir * n + sv] : 0.0;
+#pragma unroll
+  for (int j = 1; j < I; j++) {
+    const double kj = row ? kst[(j - 1) * mn + (size_t)ir * n + sv] : 0.0;
+    r = __builtin_fma(rodas_c(I, j), kj, r);
+    if (I < 6) z = __builtin_fma(rodas_a(I + 1, j), kj, z);
+  }
+  const double x = solve(a, pos, perm, rd, GAMMA * r, i, base);
+  if (I < 6 && active) {
+    kst[(I - 1) * mn + (size_t)i * n + s] = x;
+    ys[(size_t)i * n + s] = __builtin_fma(rodas_a(I + 1, I), x, z);
+  }
+  return x;
+}
+#else
+#define GAMMA D_ROS
+#define LAST_PHASE 3
+#define ERROR_ORDER 3.0  // the error estimate's local error is O(h^3)
+#define FACTOR_MAX 5.0
+#endif
+
 extern "C" __global__ void __launch_bounds__(64)
 rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
            const double* const* __restrict__ rep_t, const double* __restrict__ F,
-           double* __restrict__ y,
-           double* __restrict__ ys, double* __restrict__ k1, double* __restrict__ k2, double* __restrict__ F1,
+           double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
            double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
            int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
   const int lane = threadIdx.x;
@@ -150,13 +212,15 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
   const int ir = row ? i : 0;
 
   if (phase == 1) {
-    // W = I - h d J, factored with partial pivoting
+    // W = I - h γ J, factored with partial pivoting
 #ifdef JAC_DIFFERENCES
     const double* Fi = rep_f[ir] + (size_t)sv * COPIES;  // row i of the system's copies
     const double f0 = row ? Fi[0] : 0.0;
     const double fT = row ? rep_t[ir][sv] : 0.0;  // row i at the raised temperature
     const double rinc_i = row ? 1.0 / increment(y[(size_t)ir * n + sv], i, rtol, atol0, atol) : 0.0;
+#ifndef RODAS
     if (active) jac[(size_t)i * n + s] = f0;  // f(y), for phase 3
+#endif
 #else
     const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
 #endif
@@ -167,7 +231,7 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
 #else
       const double Jij = jac[(size_t)(M + ir * M + j) * n + sv];
 #endif
-      a[j] = (row ? -hs * D_ROS * Jij : 0.0) + (row && i == j ? 1.0 : 0.0);
+      a[j] = (row ? -hs * GAMMA * Jij : 0.0) + (row && i == j ? 1.0 : 0.0);
     }
     double diag = 1.0;  // this row's pivot, U_i[pos]
 #pragma unroll
@@ -225,15 +289,23 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
       for (int j = 0; j < M; j++) lu_s[j * M + i] = a[j];
       rd_s[i] = rd;
     }
-    const double x = solve(a, pos, perm, rd, f0, i, base);
+#ifdef RODAS
+    const double x = solve(a, pos, perm, rd, GAMMA * hs * f0, i, base);  // k1
     if (active) {
-      k1[(size_t)i * n + s] = x;
+      kst[(size_t)i * n + s] = x;
+      ys[(size_t)i * n + s] = __builtin_fma(rodas_a(2, 1), x, y[(size_t)i * n + s]);
+    }
+#else
+    const double x = solve(a, pos, perm, rd, f0, i, base);  // k1
+    if (active) {
+      kst[(size_t)i * n + s] = x;
       ys[(size_t)i * n + s] = y[(size_t)i * n + s] + 0.5 * hs * x;
     }
+#endif
     return;
   }
 
-  // phases 2 and 3: the factors and each row's pivot step back from memory
+  // later phases: the factors and each row's pivot step back from memory
 #pragma unroll
   for (int j = 0; j < M; j++) a[j] = row ? lu_s[j * M + ir] : 0.0;
   const double rd = row ? rd_s[ir] : 1.0;
@@ -246,7 +318,23 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
 #endif
     if (row && perm[k] == i) pos = k;
   }
-  const double x1 = row ? k1[(size_t)ir * n + sv] : 0.0;
+#ifdef RODAS
+  switch (phase) {
+    case 2: rodas_stage<2>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
+    case 3: rodas_stage<3>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
+    case 4: rodas_stage<4>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
+    case 5: rodas_stage<5>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
+    default: break;
+  }
+  // phase 6: y_new = Y_6 + k6, and k6 is the error estimate
+  const double k6 = rodas_stage<6>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs);
+  const double y0 = row ? y[(size_t)ir * n + sv] : 0.0;
+  const double y1 = (row ? ys[(size_t)ir * n + sv] : 0.0) + k6;
+  const double err = k6;
+#else
+  const double x1 = row ? kst[(size_t)ir * n + sv] : 0.0;
+  double* k2 = kst + (size_t)M * n;
+  double* F1 = kst + 2 * (size_t)M * n;
   if (phase == 2) {
     const double f1 = row ? F[(size_t)ir * n + sv] : 0.0;
     const double x = solve(a, pos, perm, rd, f1 - x1, i, base) + x1;
@@ -264,13 +352,15 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
   const double x3 = solve(a, pos, perm, rd, f2 - E32 * (x2 - f1) - 2.0 * (x1 - f0), i, base);
   const double y0 = row ? y[(size_t)ir * n + sv] : 0.0, y1 = row ? ys[(size_t)ir * n + sv] : 0.0;
   const double err = hs * (x1 - 2.0 * x2 + x3) * (1.0 / 6.0);
+#endif
+  // step control: accept when the scaled error is at most 1; the next h from it
   const double scale = (i == 0 ? atol0 : atol) + rtol * fmax(fabs(y0), fabs(y1));
   double ratio = row ? fabs(err) / scale : 0.0;
   if (!(ratio == ratio)) ratio = 1e300;  // a NaN (a singular W, say) rejects the step
   ratio = system_max(ratio, base);
   if (!active) return;
   const bool accept = ratio <= 1.0;
-  const double factor = fmin(5.0, fmax(0.2, 0.9 * pow(fmax(ratio, 1e-12), -1.0 / 3.0)));
+  const double factor = fmin(FACTOR_MAX, fmax(0.2, 0.9 * pow(fmax(ratio, 1e-12), -1.0 / ERROR_ORDER)));
   double t_new = t[s];
   if (accept) {
     y[(size_t)i * n + s] = y1;

@@ -421,8 +421,8 @@ class Module:
 
 
 class Integrator:
-    """cb_integrator_*: batched constant-pressure reactors on the GPU (Rosenbrock ode23s; the Jacobian analytic or by
-    difference quotients).
+    """cb_integrator_*: batched constant-pressure reactors on the GPU (Rosenbrock ode23s or RODAS4; the Jacobian
+    analytic or by difference quotients).
     advance(y, P, t, h, status, steps, t_end) on contiguous torch device tensors: y float64 [K+1][n] (T, Y_k), P, t, h
     float64 [n], status int32 [n], steps int32 [n]; returns the batch's step count."""
 
@@ -436,20 +436,23 @@ class Integrator:
             return jacobian
         return "analytic" if mechanism.K + 1 <= cls.ANALYTIC_MAX_STATE else "differences"
 
-    @classmethod
-    def flags(cls, jacobian="automatic", one_per_wave=False):
-        """cb_integrator_flags from a Jacobian name and the Rosenbrock packing."""
-        return cls.JACOBIAN[jacobian] | (0x10 if one_per_wave else 0)
+    METHOD = {"rodas4": 0, "ode23s": 0x20}
 
-    def __init__(self, mechanism, rtol=1e-6, atol_T=1e-6, atol_Y=1e-12, jacobian="automatic", one_per_wave=False):
+    @classmethod
+    def flags(cls, jacobian="automatic", one_per_wave=False, method="rodas4"):
+        """cb_integrator_flags from a Jacobian name, the Rosenbrock packing and the method."""
+        return cls.JACOBIAN[jacobian] | (0x10 if one_per_wave else 0) | cls.METHOD[method]
+
+    def __init__(self, mechanism, rtol=1e-6, atol_T=1e-6, atol_Y=1e-12, jacobian="automatic", one_per_wave=False, method="rodas4"):
         h = _P()
-        _check(lib().cb_integrator_create(mechanism._h, rtol, atol_T, atol_Y, self.flags(jacobian, one_per_wave), C.byref(h)))
+        _check(lib().cb_integrator_create(mechanism._h, rtol, atol_T, atol_Y, self.flags(jacobian, one_per_wave, method),
+                                          C.byref(h)))
         self._h = h
         self.mechanism = mechanism
 
     @classmethod
-    def compile(cls, mechanism, jacobian="automatic", one_per_wave=False):
-        _check(lib().cb_integrator_create(mechanism._h, 1e-6, 1e-6, 1e-12, cls.flags(jacobian, one_per_wave), None))
+    def compile(cls, mechanism, jacobian="automatic", one_per_wave=False, method="rodas4"):
+        _check(lib().cb_integrator_create(mechanism._h, 1e-6, 1e-6, 1e-12, cls.flags(jacobian, one_per_wave, method), None))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

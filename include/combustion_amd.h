@@ -188,10 +188,11 @@ void cb_cpu_module_free(cb_cpu_module* mod);
 
 /* Batched stiff integration of constant-pressure reactors on the GPU, the counterpart of the reference's CVODE
  * wrapper (cvode/lib.rs:17-47: CVodeCreate(CV_BDF), CVodeSStolerances(rtol, atol), dense linear solver, one step at
- * a time) and of the reactor loop of plot.rs:80-118, for many reactors at once: the L-stable (2,3) Rosenbrock pair
- * of ode23s (Shampine & Reichelt 1997) with the Jacobian of cb_build_reactor's program (analytic, cb_build_jacobian,
- * or difference quotients: flags) and floor(64/(K+1)) reactors per wave for the LU factorisation and the solves
- * (state size K+1 <= 64). Tolerances: rtol, and
+ * a time) and of the reactor loop of plot.rs:80-118, for many reactors at once: the stiffly accurate order-4
+ * Rosenbrock method RODAS4 (Hairer & Wanner, with its embedded order-3 estimate; or, with CB_INTEGRATOR_ODE23S, the
+ * L-stable (2,3) pair of ode23s, Shampine & Reichelt 1997) with the Jacobian of cb_build_reactor's program (analytic,
+ * cb_build_jacobian, or difference quotients: flags) and floor(64/(K+1)) reactors per wave for the LU factorisation
+ * and the solves (state size K+1 <= 64). Tolerances: rtol, and
  * atol_T (temperature) / atol_Y (mass fractions). With out == NULL the kernels are only compiled into the cache.
  * cb_integrator_advance moves n reactors toward t_end, in place, every array in device memory: y [K+1][n] (SoA rows
  * T, Y_0..Y_{K-1}), P [n] (constant), t [n] and h [n] (each reactor's time and next step size), status [n] (0
@@ -205,7 +206,8 @@ enum cb_integrator_flags {
   CB_JACOBIAN_DIFFERENCES = 2, /* difference quotients, as CVODE's dense solver forms them: every state copied K+2
                                   times (one copy per perturbed component, inc_j = sqrt(eps) max(|y_j|, atol_j/rtol)),
                                   one right-hand-side launch over the copies, the quotients formed by the LU kernel */
-  CB_INTEGRATOR_ONE_PER_WAVE = 0x10 /* one reactor per Rosenbrock wave whatever its size (otherwise floor(64/(K+1))) */
+  CB_INTEGRATOR_ONE_PER_WAVE = 0x10, /* one reactor per Rosenbrock wave whatever its size (otherwise floor(64/(K+1))) */
+  CB_INTEGRATOR_ODE23S = 0x20 /* ode23s's L-stable (2,3) pair (three stages) instead of RODAS4 */
 };
 int cb_integrator_create(const cb_mechanism* m, double rtol, double atol_T, double atol_Y, int32_t flags, cb_integrator** out);
 int cb_integrator_advance(cb_integrator* it, size_t n_reactors, double* y, const double* P, double* t, double* h, int32_t* status,

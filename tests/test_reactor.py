@@ -54,12 +54,13 @@ def test_integrator_options_are_checked_before_any_compilation():
     own errors, before it compiles or touches a device (the out == NULL call only fills the code-object cache)."""
     from combustion_amd import Mechanism, native
     M, mech, o = setup("h2o2-10")
-    for flags in (0x3, 0x20, 0x100):
+    for flags in (0x3, 0x40, 0x100):
         with pytest.raises(native.Error) as e:
             native._check(native.lib().cb_integrator_create(M._h, 1e-6, 1e-6, 1e-12, flags, None))
         assert e.value.kind == "argument"
     assert native.Integrator.flags("differences", True) == 0x12
-    assert native.Integrator.flags() == 0
+    assert native.Integrator.flags("analytic", method="ode23s") == 0x21
+    assert native.Integrator.flags() == 0  # automatic Jacobian, packed waves, RODAS4
     big = Mechanism(synth.mechanism_path("large-160"))
     with pytest.raises(native.Error) as e:
         native.Integrator.compile(big)
