@@ -240,7 +240,8 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   if (n == 0) return 0;
   if (n >= (size_t(1) << 31)) throw ArgumentError("at most 2^31 - 1 reactors");
   prepare(y, P, n);
-  hipFunction_t running = nullptr, clear = nullptr, copies_kernel = nullptr;
+  hipFunction_t running = nullptr, clear = nullptr, copies_kernel = nullptr, stage = nullptr;
+  device::check(hipModuleGetFunction(&stage, step_.module, "rosenbrock_stage"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
   if (differences_) device::check(hipModuleGetFunction(&copies_kernel, step_.module, "rosenbrock_copies"), "hipModuleGetFunction");
@@ -265,8 +266,8 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
     for (int phase = 1; phase <= phases_; phase++) {
       void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &kst_,
                       &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
-      device::check(hipModuleLaunchKernel(step_.function, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1, 1, 0, stream, args,
-                                          nullptr),
+      device::check(hipModuleLaunchKernel(phase == 1 ? step_.function : stage, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1,
+                                          1, 0, stream, args, nullptr),
                     "launch rosenbrock");
       if (phase < phases_) rhs_->launch_only((const double* const*)tab_ys_, (double* const*)tab_F_, n, stream, nullptr);
     }

@@ -172,12 +172,15 @@ __device__ __forceinline__ double rodas_stage(const double (&a)[M], int pos, con
 #define FACTOR_MAX 5.0
 #endif
 
-extern "C" __global__ void __launch_bounds__(64)
-rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
-           const double* const* __restrict__ rep_t, const double* __restrict__ F,
-           double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
-           double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
-           int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
+// Phase 1 (FIRST) or a later phase (phase >= 2): two kernels, so that the later phases, which hold no LU in
+// progress, are compiled to their own (smaller) register count.
+template <bool FIRST>
+__device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+                                                 const double* const* __restrict__ rep_t, const double* __restrict__ F,
+                                                 double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
+                                                 double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t,
+                                                 double* __restrict__ h, int* __restrict__ status, unsigned* __restrict__ steps,
+                                                 double t_end, double rtol, double atol0, double atol) {
   const int lane = threadIdx.x;
   const int sys = lane / M, i = lane - sys * M, base = sys * M;
   const bool row = sys < P;  // lanes past P M hold no row (with P = 1: the lanes past M)
@@ -191,7 +194,7 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
   bool active = valid && status[sv] == 0;
 #endif
   double hs = h[sv];
-  if (phase == 1) {
+  if (FIRST) {
     const double remaining = t_end - t[sv];
     if (active && remaining <= 1e-12 * fabs(t_end)) {
       if (i == 0) status[s] = 1;
@@ -201,7 +204,7 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
     if (active && i == 0) h[s] = hs;
   }
 #if P == 1
-  if (phase == 1 && !__builtin_amdgcn_readfirstlane((int)active)) return;  // the system is at t_end (lane 0 is a row)
+  if (FIRST && !__builtin_amdgcn_readfirstlane((int)active)) return;  // the system is at t_end (lane 0 is a row)
 #endif
   int* perm_s = perm_all + (size_t)sv * M;
   double* lu_s = lu + (size_t)sv * M * M;
@@ -211,7 +214,7 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
   int pos = -1;
   const int ir = row ? i : 0;
 
-  if (phase == 1) {
+  if (FIRST) {
     // W = I - h γ J, factored with partial pivoting
 #ifdef JAC_DIFFERENCES
     const double* Fi = rep_f[ir] + (size_t)sv * COPIES;  // row i of the system's copies
@@ -379,6 +382,25 @@ rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const*
       h[s] = h_new;
     }
   }
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+           const double* const* __restrict__ rep_t, const double* __restrict__ F,
+           double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
+           double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
+           int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
+  rosenbrock_phase<true>(phase, n, jac, rep_f, rep_t, F, y, ys, kst, lu, perm_all, t, h, status, steps, t_end, rtol, atol0, atol);
+}
+
+// Phases 2 and later (phase >= 2).
+extern "C" __global__ void __launch_bounds__(64)
+rosenbrock_stage(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+                 const double* const* __restrict__ rep_t, const double* __restrict__ F,
+                 double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
+                 double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
+                 int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
+  rosenbrock_phase<false>(phase, n, jac, rep_f, rep_t, F, y, ys, kst, lu, perm_all, t, h, status, steps, t_end, rtol, atol0, atol);
 }
 
 #ifdef JAC_DIFFERENCES
