@@ -3,10 +3,11 @@
 // fractions — its temperature and pressure part once per reactor, the rest over the reactor's K+1 copies, each copy
 // raising one component — and the whole right-hand side at the raised temperature, whose outputs the first
 // Rosenbrock phase turns into J), the Rosenbrock phases of kernels/rosenbrock.hip (three for ode23s, six for
-// RODAS4) with a right-hand-side evaluation between two phases, all on the caller's stream; the batch is polled for
-// reactors still running every few steps.
+// RODAS4) with a right-hand-side evaluation between two phases; the batch in two parts, on the caller's stream and
+// on a stream of the integrator's own, polled for reactors still running every few steps.
 #include "integrator.hpp"
 
+#include <algorithm>
 #include <sstream>
 
 #include "builders.hpp"
@@ -161,11 +162,17 @@ Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double a
   step_.load(device::compile(p.step_source, "rosenbrock", hip_options(true)), "rosenbrock");
 }
 
-Integrator::~Integrator() { release(); }
+Integrator::~Integrator() {
+  release();
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
+  if (side_) (void)hipStreamDestroy(side_);
+}
 
 void Integrator::release() {
   for (void* p : buffers_) (void)hipFree(p);
   buffers_.clear();
+  parts_.clear();
   n_ = 0;
   last_y_ = nullptr;
   last_P_ = nullptr;
@@ -175,6 +182,11 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   if (n == n_ && y == last_y_ && P == last_P_) return;
   device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   release();
+  if (!side_) {
+    device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    device::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
+    device::check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreateWithFlags");
+  }
   const size_t m = (size_t)m_;
   auto alloc = [&](size_t bytes) {
     void* p = nullptr;
@@ -186,49 +198,61 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   F_ = (double*)alloc(m * n * sizeof(double));
   ys_ = (double*)alloc(m * n * sizeof(double));
   kst_ = (double*)alloc((phases_ == 6 ? 5 : 3) * m * n * sizeof(double));  // RODAS4: k1..k5; ode23s: k1, k2, F1
-  lu_ = (double*)alloc((m * m + m) * n * sizeof(double));  // the factors, then the reciprocal pivots
+  lu_ = (double*)alloc(m * m * n * sizeof(double));
+  rd_ = (double*)alloc(m * n * sizeof(double));
   perm_ = (int*)alloc(m * n * sizeof(int));
   running_ = (unsigned*)alloc(256);
-  // pointer tables of the two programs: inputs [T, P, Y_0..Y_{K-1}] from y or ys; outputs F rows or jac rows
+  const size_t B = differences_ ? shared_linear_.output.size() : 0, C = copies_per_reactor(m_), nr = n * C;
+  double *rep_out = nullptr, *t_out = nullptr;
+  if (differences_) {
+    // the reactors' values for their copies (group_stride per reactor); the right-hand sides of the C copies of
+    // every state (C the power of two >= K + 1; M rows of C n doubles); the temperature-raised states' outputs
+    groups_ = (double*)alloc(group_stride(B, m_) * n * sizeof(double));
+    rep_out = (double*)alloc(m * nr * sizeof(double));
+    t_out = (double*)alloc(m * n * sizeof(double));
+    raised_T_ = (double*)alloc(n * sizeof(double));
+  }
+  // the batch in two parts, each on its own stream; the first part's length a multiple of 8 (64-byte rows)
+  const size_t first = std::min(n, ((n + 1) / 2 + 7) & ~(size_t)7);
   auto table = [&](const std::vector<const void*>& ptrs) {
     void** t = (void**)alloc(ptrs.size() * sizeof(void*));
     device::check(hipMemcpy(t, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice), "hipMemcpy table");
     return t;
   };
-  auto inputs = [&](const double* base) {
-    std::vector<const void*> p = {base, P};
-    for (size_t i = 1; i < m; i++) p.push_back(base + i * n);
-    return p;
-  };
-  auto rows = [&](const double* base, size_t count) {
-    std::vector<const void*> p;
-    for (size_t i = 0; i < count; i++) p.push_back(base + i * n);
-    return p;
-  };
-  tab_y_ = table(inputs(y));
-  tab_ys_ = table(inputs(ys_));
-  if (!differences_) tab_jac_ = table(rows(jac_, m + m * m));
-  tab_F_ = table(rows(F_, m));
-  if (differences_) {
-    // the reactors' values for their copies (group_stride per reactor); the right-hand sides of the C copies of
-    // every state (C the power of two >= K + 1; M rows of C n doubles); the temperature-raised states (T row, then P
-    // and y's rows)
-    const size_t B = shared_linear_.output.size(), nr = n * copies_per_reactor(m_);
-    groups_ = (double*)alloc(group_stride(B, m_) * n * sizeof(double));
-    double* rep_out = (double*)alloc(m * nr * sizeof(double));
-    double* t_out = (double*)alloc(m * n * sizeof(double));
-    raised_T_ = (double*)alloc(n * sizeof(double));
-    std::vector<const void*> out_rows, shared_out;
-    for (size_t i = 0; i < m; i++) out_rows.push_back(rep_out + i * nr);
-    for (size_t b = 0; b < B; b++) shared_out.push_back(groups_ + b);
-    tab_shared_in_ = table({y, P});
-    tab_shared_out_ = table(shared_out);
-    tab_rep_in_ = table({groups_});
-    tab_rep_out_ = table(out_rows);
-    std::vector<const void*> t_in = inputs(y);
-    t_in[0] = raised_T_;
-    tab_t_in_ = table(t_in);
-    tab_t_out_ = table(rows(t_out, m));
+  for (size_t off : {size_t(0), first}) {
+    Part q;
+    q.off = off;
+    q.n = off == 0 ? first : n - first;
+    if (q.n == 0) continue;
+    // pointer tables of the programs: inputs [T, P, Y_0..Y_{K-1}] from y or ys; outputs F rows or jac rows; all
+    // starting at the part's first reactor
+    auto inputs = [&](const double* base) {
+      std::vector<const void*> p = {base + off, P + off};
+      for (size_t i = 1; i < m; i++) p.push_back(base + i * n + off);
+      return p;
+    };
+    auto rows = [&](const double* base, size_t count, size_t ld, size_t first_value) {
+      std::vector<const void*> p;
+      for (size_t i = 0; i < count; i++) p.push_back(base + i * ld + first_value);
+      return p;
+    };
+    q.tab_y = table(inputs(y));
+    q.tab_ys = table(inputs(ys_));
+    if (!differences_) q.tab_jac = table(rows(jac_, m + m * m, n, off));
+    q.tab_F = table(rows(F_, m, n, off));
+    if (differences_) {
+      std::vector<const void*> shared_out;
+      for (size_t b = 0; b < B; b++) shared_out.push_back(groups_ + off * group_stride(B, m_) + b);
+      q.tab_shared_in = table({y + off, P + off});
+      q.tab_shared_out = table(shared_out);
+      q.tab_rep_in = table({groups_ + off * group_stride(B, m_)});
+      q.tab_rep_out = table(rows(rep_out, m, nr, off * C));
+      std::vector<const void*> t_in = inputs(y);
+      t_in[0] = raised_T_ + off;
+      q.tab_t_in = table(t_in);
+      q.tab_t_out = table(rows(t_out, m, n, off));
+    }
+    parts_.push_back(q);
   }
   n_ = n;
   last_y_ = y;
@@ -246,33 +270,53 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   device::check(hipModuleGetFunction(&clear, step_.module, "rosenbrock_clear"), "hipModuleGetFunction");
   if (differences_) device::check(hipModuleGetFunction(&copies_kernel, step_.module, "rosenbrock_copies"), "hipModuleGetFunction");
   unsigned n32 = (unsigned)n;
+  size_t ld = n;
   const unsigned systems_per_wave = systems_per_wave_;
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
-  const size_t copies = n * (size_t)copies_per_reactor(m_);
+  const size_t m = (size_t)m_, C = copies_per_reactor(m_);
   unsigned stride = (unsigned)group_stride(shared_linear_.output.size(), m_), shared_values = (unsigned)shared_linear_.output.size();
+  // the second part's stream starts after the work already on the caller's stream
+  device::check(hipEventRecord(fork_, stream), "hipEventRecord");
+  device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
   long taken = 0;
   for (int step = 0; step < max_steps; step++) {
-    if (differences_) {
-      void* rargs[] = {&n32, &y, &P, &groups_, &stride, &shared_values, &raised_T_, &rtol, &atol0, &atol};
-      const size_t fields = n * 2 * (size_t)m_;
-      device::check(hipModuleLaunchKernel(copies_kernel, (unsigned)((fields + 255) / 256), 1, 1, 256, 1, 1, 0, stream, rargs, nullptr),
-                    "launch rosenbrock_copies");
-      shared_->launch_only((const double* const*)tab_shared_in_, (double* const*)tab_shared_out_, n, stream, nullptr);
-      varying_->launch_only((const double* const*)tab_rep_in_, (double* const*)tab_rep_out_, copies, stream, nullptr);
-      rhs_->launch_only((const double* const*)tab_t_in_, (double* const*)tab_t_out_, n, stream, nullptr);
-    } else {
-      jacobian_->launch_only((const double* const*)tab_y_, (double* const*)tab_jac_, n, stream, nullptr);
-    }
-    for (int phase = 1; phase <= phases_; phase++) {
-      void* args[] = {&phase, &n32, &jac_, &tab_rep_out_, &tab_t_out_, &F_, &y, &ys_, &kst_,
-                      &lu_, &perm_, &t, &h, &status, &steps, &te, &rtol, &atol0, &atol};
-      device::check(hipModuleLaunchKernel(phase == 1 ? step_.function : stage, (n32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64, 1,
-                                          1, 0, stream, args, nullptr),
-                    "launch rosenbrock");
-      if (phase < phases_) rhs_->launch_only((const double* const*)tab_ys_, (double* const*)tab_F_, n, stream, nullptr);
+    for (size_t p = 0; p < parts_.size(); p++) {
+      const Part& q = parts_[p];
+      hipStream_t st = p == 0 ? stream : side_;
+      const size_t off = q.off, count = q.n;
+      unsigned c32 = (unsigned)count;
+      if (differences_) {
+        double *yq = y + off, *groups = groups_ + off * stride, *raised = raised_T_ + off;
+        const double* Pq = P + off;
+        void* rargs[] = {&c32, &ld, &yq, &Pq, &groups, &stride, &shared_values, &raised, &rtol, &atol0, &atol};
+        const size_t fields = count * 2 * m;
+        device::check(hipModuleLaunchKernel(copies_kernel, (unsigned)((fields + 255) / 256), 1, 1, 256, 1, 1, 0, st, rargs, nullptr),
+                      "launch rosenbrock_copies");
+        shared_->launch_only((const double* const*)q.tab_shared_in, (double* const*)q.tab_shared_out, count, st, nullptr);
+        varying_->launch_only((const double* const*)q.tab_rep_in, (double* const*)q.tab_rep_out, count * C, st, nullptr);
+        rhs_->launch_only((const double* const*)q.tab_t_in, (double* const*)q.tab_t_out, count, st, nullptr);
+      } else {
+        jacobian_->launch_only((const double* const*)q.tab_y, (double* const*)q.tab_jac, count, st, nullptr);
+      }
+      double *jac = jac_ + off, *F = F_ + off, *yq = y + off, *ys = ys_ + off, *kst = kst_ + off, *lu = lu_ + off * m * m,
+             *rd = rd_ + off * m, *tq = t + off, *hq = h + off;
+      int *perm = perm_ + off * m, *statq = status + off;
+      unsigned* stepq = steps + off;
+      void **rep_f = q.tab_rep_out, **rep_t = q.tab_t_out;
+      for (int phase = 1; phase <= phases_; phase++) {
+        void* args[] = {&phase, &c32, &ld, &jac, &rep_f, &rep_t, &F, &yq, &ys, &kst,
+                        &lu, &rd, &perm, &tq, &hq, &statq, &stepq, &te, &rtol, &atol0, &atol};
+        device::check(hipModuleLaunchKernel(phase == 1 ? step_.function : stage, (c32 + systems_per_wave - 1) / systems_per_wave, 1, 1, 64,
+                                            1, 1, 0, st, args, nullptr),
+                      "launch rosenbrock");
+        if (phase < phases_) rhs_->launch_only((const double* const*)q.tab_ys, (double* const*)q.tab_F, count, st, nullptr);
+      }
     }
     taken++;
     if ((step + 1) % kPollEvery == 0 || step + 1 == max_steps) {
+      // the count waits for both parts' steps so far
+      device::check(hipEventRecord(join_, side_), "hipEventRecord");
+      device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
       void* cargs[] = {&running_};
       device::check(hipModuleLaunchKernel(clear, 1, 1, 1, 64, 1, 1, 0, stream, cargs, nullptr), "launch rosenbrock_clear");
       void* rargs[] = {&n32, &status, &running_};

@@ -53,15 +53,24 @@ class Integrator {
   device::Kernel step_;
   size_t n_ = 0;
   std::vector<void*> buffers_;
-  double *jac_ = nullptr, *F_ = nullptr, *ys_ = nullptr, *kst_ = nullptr, *lu_ = nullptr;  // kst_: the stage vectors
+  double *jac_ = nullptr, *F_ = nullptr, *ys_ = nullptr, *kst_ = nullptr, *lu_ = nullptr, *rd_ = nullptr;  // kst_: the stage vectors
   int* perm_ = nullptr;
   unsigned* running_ = nullptr;
-  void **tab_y_ = nullptr, **tab_ys_ = nullptr, **tab_jac_ = nullptr, **tab_F_ = nullptr;
-  // difference quotients: the shared part's inputs [T, P] and outputs; the copies' inputs (the reactors' values)
-  // and outputs; the temperature-raised states' inputs and outputs
-  void **tab_shared_in_ = nullptr, **tab_shared_out_ = nullptr, **tab_rep_in_ = nullptr, **tab_rep_out_ = nullptr;
-  void **tab_t_in_ = nullptr, **tab_t_out_ = nullptr;
   double *groups_ = nullptr, *raised_T_ = nullptr;  // each reactor's values for its copies; the raised temperatures
+  // The batch runs in two parts, the first on the caller's stream and the second on side_, so that one part's
+  // latency-bound right-hand sides overlap the other's Rosenbrock phases. Each part's pointer tables start at its
+  // first reactor: the programs' inputs from y or ys and outputs to F or jac; for difference quotients, the shared
+  // part's inputs [T, P] and outputs, the copies' inputs (the reactors' values) and outputs, and the
+  // temperature-raised states' inputs and outputs.
+  struct Part {
+    size_t off = 0, n = 0;
+    void **tab_y = nullptr, **tab_ys = nullptr, **tab_jac = nullptr, **tab_F = nullptr;
+    void **tab_shared_in = nullptr, **tab_shared_out = nullptr, **tab_rep_in = nullptr, **tab_rep_out = nullptr;
+    void **tab_t_in = nullptr, **tab_t_out = nullptr;
+  };
+  std::vector<Part> parts_;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
   const double* last_y_ = nullptr;
   const double* last_P_ = nullptr;
   void release();

@@ -23,10 +23,11 @@
 // Systems whose status is not 0 (running) are left alone. Blocks take their systems in XCD-aware order: the 8 XCDs
 // receive blocks in turn, and block b works on system group xcd·⌊B/8⌋ + min(xcd, B mod 8) + ⌊b/8⌋ (B blocks), so
 // the systems that share a cache line of the SoA vectors run under one L2.
-// Layouts: vectors are SoA, component i of system s at [i * n + s]; jac holds f(y) in rows 0..M-1 and J row-major in
-// rows M..M+M²-1 (the Jacobian program's outputs); lu[s][j][i] (column-major by system) then the M reciprocal pivots
-// of system s at lu[n M² + s M + i]; perm[s][k] = the row chosen at step k; kst holds the stage vectors, M n doubles
-// each (ode23s: k1, k2, F1; RODAS4: k1..k5).
+// Layouts: a launch covers systems 0..n-1 of pointers that start at its first system; vectors are SoA with row
+// stride ld >= n, component i of system s at [i * ld + s]; jac holds f(y) in rows 0..M-1 and J row-major in rows
+// M..M+M²-1 (the Jacobian program's outputs); lu[s][j][i] (column-major by system), rdp[s][i] the reciprocal pivots;
+// perm[s][k] = the row chosen at step k; kst holds the stage vectors, M ld doubles each (ode23s: k1, k2, F1;
+// RODAS4: k1..k5).
 // With JAC_DIFFERENCES defined (and COPIES, a power of two >= M), J is formed in phase 1 from difference quotients
 // instead: the right-hand side is evaluated for COPIES copies of every state, system by system (copy 0 and copies past
 // M-1 the state, copy j = 1..M-1 with component j raised by inc_j, from the values rosenbrock_copies writes), and at
@@ -146,22 +147,22 @@ __device__ __forceinline__ constexpr double rodas_c(int i, int j) {
 // stores k_I and the next stage point Y_{I+1} = y + Σ_{j<=I} a_{I+1,j} k_j in ys; returns k_I.
 template <int I>
 __device__ __forceinline__ double rodas_stage(const double (&a)[M], int pos, const int (&perm)[M], double rd, int i, int base, bool row,
-                                              bool active, unsigned n, unsigned s, unsigned sv, const double* __restrict__ F,
+                                              bool active, size_t ld, unsigned s, unsigned sv, const double* __restrict__ F,
                                               const double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst, double hs) {
   const int ir = row ? i : 0;
-  const size_t mn = (size_t)M * n;
-  double r = row ? hs * F[(size_t)ir * n + sv] : 0.0;
-  double z = I < 6 && row ? y[(size_t)ir * n + sv] : 0.0;
+  const size_t mn = (size_t)M * ld;
+  double r = row ? hs * F[(size_t)ir * ld + sv] : 0.0;
+  double z = I < 6 && row ? y[(size_t)ir * ld + sv] : 0.0;
 #pragma unroll
   for (int j = 1; j < I; j++) {
-    const double kj = row ? kst[(j - 1) * mn + (size_t)ir * n + sv] : 0.0;
+    const double kj = row ? kst[(j - 1) * mn + (size_t)ir * ld + sv] : 0.0;
     r = __builtin_fma(rodas_c(I, j), kj, r);
     if (I < 6) z = __builtin_fma(rodas_a(I + 1, j), kj, z);
   }
   const double x = solve(a, pos, perm, rd, GAMMA * r, i, base);
   if (I < 6 && active) {
-    kst[(I - 1) * mn + (size_t)i * n + s] = x;
-    ys[(size_t)i * n + s] = __builtin_fma(rodas_a(I + 1, I), x, z);
+    kst[(I - 1) * mn + (size_t)i * ld + s] = x;
+    ys[(size_t)i * ld + s] = __builtin_fma(rodas_a(I + 1, I), x, z);
   }
   return x;
 }
@@ -175,10 +176,10 @@ __device__ __forceinline__ double rodas_stage(const double (&a)[M], int pos, con
 // Phase 1 (FIRST) or a later phase (phase >= 2): two kernels, so that the later phases, which hold no LU in
 // progress, are compiled to their own (smaller) register count.
 template <bool FIRST>
-__device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+__device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, size_t ld, double* __restrict__ jac, const double* const* __restrict__ rep_f,
                                                  const double* const* __restrict__ rep_t, const double* __restrict__ F,
                                                  double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
-                                                 double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t,
+                                                 double* __restrict__ lu, double* __restrict__ rdp, int* __restrict__ perm_all, double* __restrict__ t,
                                                  double* __restrict__ h, int* __restrict__ status, unsigned* __restrict__ steps,
                                                  double t_end, double rtol, double atol0, double atol) {
   const int lane = threadIdx.x;
@@ -208,7 +209,7 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* 
 #endif
   int* perm_s = perm_all + (size_t)sv * M;
   double* lu_s = lu + (size_t)sv * M * M;
-  double* rd_s = lu + (size_t)n * M * M + (size_t)sv * M;
+  double* rd_s = rdp + (size_t)sv * M;
   double a[M];
   int perm[M];  // the same in every lane of a system (wave-uniform when P = 1)
   int pos = -1;
@@ -220,19 +221,19 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* 
     const double* Fi = rep_f[ir] + (size_t)sv * COPIES;  // row i of the system's copies
     const double f0 = row ? Fi[0] : 0.0;
     const double fT = row ? rep_t[ir][sv] : 0.0;  // row i at the raised temperature
-    const double rinc_i = row ? 1.0 / increment(y[(size_t)ir * n + sv], i, rtol, atol0, atol) : 0.0;
+    const double rinc_i = row ? 1.0 / increment(y[(size_t)ir * ld + sv], i, rtol, atol0, atol) : 0.0;
 #ifndef RODAS
-    if (active) jac[(size_t)i * n + s] = f0;  // f(y), for phase 3
+    if (active) jac[(size_t)i * ld + s] = f0;  // f(y), for phase 3
 #endif
 #else
-    const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
+    const double f0 = row ? jac[(size_t)ir * ld + sv] : 0.0;
 #endif
 #pragma unroll
     for (int j = 0; j < M; j++) {
 #ifdef JAC_DIFFERENCES
       const double Jij = ((j == 0 ? fT : Fi[j]) - f0) * row_value(rinc_i, j, base);
 #else
-      const double Jij = jac[(size_t)(M + ir * M + j) * n + sv];
+      const double Jij = jac[(size_t)(M + ir * M + j) * ld + sv];
 #endif
       a[j] = (row ? -hs * GAMMA * Jij : 0.0) + (row && i == j ? 1.0 : 0.0);
     }
@@ -295,14 +296,14 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* 
 #ifdef RODAS
     const double x = solve(a, pos, perm, rd, GAMMA * hs * f0, i, base);  // k1
     if (active) {
-      kst[(size_t)i * n + s] = x;
-      ys[(size_t)i * n + s] = __builtin_fma(rodas_a(2, 1), x, y[(size_t)i * n + s]);
+      kst[(size_t)i * ld + s] = x;
+      ys[(size_t)i * ld + s] = __builtin_fma(rodas_a(2, 1), x, y[(size_t)i * ld + s]);
     }
 #else
     const double x = solve(a, pos, perm, rd, f0, i, base);  // k1
     if (active) {
-      kst[(size_t)i * n + s] = x;
-      ys[(size_t)i * n + s] = y[(size_t)i * n + s] + 0.5 * hs * x;
+      kst[(size_t)i * ld + s] = x;
+      ys[(size_t)i * ld + s] = y[(size_t)i * ld + s] + 0.5 * hs * x;
     }
 #endif
     return;
@@ -323,37 +324,37 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* 
   }
 #ifdef RODAS
   switch (phase) {
-    case 2: rodas_stage<2>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
-    case 3: rodas_stage<3>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
-    case 4: rodas_stage<4>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
-    case 5: rodas_stage<5>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs); return;
+    case 2: rodas_stage<2>(a, pos, perm, rd, i, base, row, active, ld, s, sv, F, y, ys, kst, hs); return;
+    case 3: rodas_stage<3>(a, pos, perm, rd, i, base, row, active, ld, s, sv, F, y, ys, kst, hs); return;
+    case 4: rodas_stage<4>(a, pos, perm, rd, i, base, row, active, ld, s, sv, F, y, ys, kst, hs); return;
+    case 5: rodas_stage<5>(a, pos, perm, rd, i, base, row, active, ld, s, sv, F, y, ys, kst, hs); return;
     default: break;
   }
   // phase 6: y_new = Y_6 + k6, and k6 is the error estimate
-  const double k6 = rodas_stage<6>(a, pos, perm, rd, i, base, row, active, n, s, sv, F, y, ys, kst, hs);
-  const double y0 = row ? y[(size_t)ir * n + sv] : 0.0;
-  const double y1 = (row ? ys[(size_t)ir * n + sv] : 0.0) + k6;
+  const double k6 = rodas_stage<6>(a, pos, perm, rd, i, base, row, active, ld, s, sv, F, y, ys, kst, hs);
+  const double y0 = row ? y[(size_t)ir * ld + sv] : 0.0;
+  const double y1 = (row ? ys[(size_t)ir * ld + sv] : 0.0) + k6;
   const double err = k6;
 #else
-  const double x1 = row ? kst[(size_t)ir * n + sv] : 0.0;
-  double* k2 = kst + (size_t)M * n;
-  double* F1 = kst + 2 * (size_t)M * n;
+  const double x1 = row ? kst[(size_t)ir * ld + sv] : 0.0;
+  double* k2 = kst + (size_t)M * ld;
+  double* F1 = kst + 2 * (size_t)M * ld;
   if (phase == 2) {
-    const double f1 = row ? F[(size_t)ir * n + sv] : 0.0;
+    const double f1 = row ? F[(size_t)ir * ld + sv] : 0.0;
     const double x = solve(a, pos, perm, rd, f1 - x1, i, base) + x1;
     if (active) {
-      F1[(size_t)i * n + s] = f1;
-      k2[(size_t)i * n + s] = x;
-      ys[(size_t)i * n + s] = y[(size_t)i * n + s] + hs * x;
+      F1[(size_t)i * ld + s] = f1;
+      k2[(size_t)i * ld + s] = x;
+      ys[(size_t)i * ld + s] = y[(size_t)i * ld + s] + hs * x;
     }
     return;
   }
-  const double x2 = row ? k2[(size_t)ir * n + sv] : 0.0;
-  const double f0 = row ? jac[(size_t)ir * n + sv] : 0.0;
-  const double f1 = row ? F1[(size_t)ir * n + sv] : 0.0;
-  const double f2 = row ? F[(size_t)ir * n + sv] : 0.0;
+  const double x2 = row ? k2[(size_t)ir * ld + sv] : 0.0;
+  const double f0 = row ? jac[(size_t)ir * ld + sv] : 0.0;
+  const double f1 = row ? F1[(size_t)ir * ld + sv] : 0.0;
+  const double f2 = row ? F[(size_t)ir * ld + sv] : 0.0;
   const double x3 = solve(a, pos, perm, rd, f2 - E32 * (x2 - f1) - 2.0 * (x1 - f0), i, base);
-  const double y0 = row ? y[(size_t)ir * n + sv] : 0.0, y1 = row ? ys[(size_t)ir * n + sv] : 0.0;
+  const double y0 = row ? y[(size_t)ir * ld + sv] : 0.0, y1 = row ? ys[(size_t)ir * ld + sv] : 0.0;
   const double err = hs * (x1 - 2.0 * x2 + x3) * (1.0 / 6.0);
 #endif
   // step control: accept when the scaled error is at most 1; the next h from it
@@ -366,7 +367,7 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* 
   const double factor = fmin(FACTOR_MAX, fmax(0.2, 0.9 * pow(fmax(ratio, 1e-12), -1.0 / ERROR_ORDER)));
   double t_new = t[s];
   if (accept) {
-    y[(size_t)i * n + s] = y1;
+    y[(size_t)i * ld + s] = y1;
     t_new += hs;
   }
   const double remaining = t_end - t_new;
@@ -385,22 +386,22 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, double* 
 }
 
 extern "C" __global__ void __launch_bounds__(64)
-rosenbrock(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+rosenbrock(int phase, unsigned n, size_t ld, double* __restrict__ jac, const double* const* __restrict__ rep_f,
            const double* const* __restrict__ rep_t, const double* __restrict__ F,
            double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
-           double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
+           double* __restrict__ lu, double* __restrict__ rdp, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
            int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
-  rosenbrock_phase<true>(phase, n, jac, rep_f, rep_t, F, y, ys, kst, lu, perm_all, t, h, status, steps, t_end, rtol, atol0, atol);
+  rosenbrock_phase<true>(phase, n, ld, jac, rep_f, rep_t, F, y, ys, kst, lu, rdp, perm_all, t, h, status, steps, t_end, rtol, atol0, atol);
 }
 
 // Phases 2 and later (phase >= 2).
 extern "C" __global__ void __launch_bounds__(64)
-rosenbrock_stage(int phase, unsigned n, double* __restrict__ jac, const double* const* __restrict__ rep_f,
+rosenbrock_stage(int phase, unsigned n, size_t ld, double* __restrict__ jac, const double* const* __restrict__ rep_f,
                  const double* const* __restrict__ rep_t, const double* __restrict__ F,
                  double* __restrict__ y, double* __restrict__ ys, double* __restrict__ kst,
-                 double* __restrict__ lu, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
+                 double* __restrict__ lu, double* __restrict__ rdp, int* __restrict__ perm_all, double* __restrict__ t, double* __restrict__ h,
                  int* __restrict__ status, unsigned* __restrict__ steps, double t_end, double rtol, double atol0, double atol) {
-  rosenbrock_phase<false>(phase, n, jac, rep_f, rep_t, F, y, ys, kst, lu, perm_all, t, h, status, steps, t_end, rtol, atol0, atol);
+  rosenbrock_phase<false>(phase, n, ld, jac, rep_f, rep_t, F, y, ys, kst, lu, rdp, perm_all, t, h, status, steps, t_end, rtol, atol0, atol);
 }
 
 #ifdef JAC_DIFFERENCES
@@ -409,7 +410,7 @@ rosenbrock_stage(int phase, unsigned n, double* __restrict__ jac, const double* 
 // (j = 1..M-1); and raised_T[s] = T + inc_0 for the whole right-hand side at the raised temperature. One thread per
 // (system, f).
 extern "C" __global__ void __launch_bounds__(256)
-rosenbrock_copies(unsigned n, const double* __restrict__ y, const double* __restrict__ P_in, double* __restrict__ groups,
+rosenbrock_copies(unsigned n, size_t ld, const double* __restrict__ y, const double* __restrict__ P_in, double* __restrict__ groups,
                   unsigned stride, unsigned shared, double* __restrict__ raised_T, double rtol, double atol0, double atol) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)n * 2 * M) return;
@@ -421,10 +422,10 @@ rosenbrock_copies(unsigned n, const double* __restrict__ y, const double* __rest
   } else if (f == 1) {
     v = P_in[s];
   } else if (f <= M) {
-    v = y[(size_t)(f - 1) * n + s];
+    v = y[(size_t)(f - 1) * ld + s];
   } else {
     const int j = (int)f - M;
-    v = increment(y[(size_t)j * n + s], j, rtol, atol0, atol);
+    v = increment(y[(size_t)j * ld + s], j, rtol, atol0, atol);
   }
   groups[(size_t)s * stride + shared + f] = v;
 }

@@ -197,7 +197,8 @@ void cb_cpu_module_free(cb_cpu_module* mod);
  * cb_integrator_advance moves n reactors toward t_end, in place, every array in device memory: y [K+1][n] (SoA rows
  * T, Y_0..Y_{K-1}), P [n] (constant), t [n] and h [n] (each reactor's time and next step size), status [n] (0
  * running; set to 1 at t_end, 2 when the step size collapsed; others are left alone), steps [n] (attempted steps).
- * It stops after max_steps steps of the batch or when no reactor runs, and blocks until then. An integrator keeps
+ * It stops after max_steps steps of the batch or when no reactor runs, and blocks until then. Half of the batch runs
+ * on a stream of the integrator's own, after the work already on `stream` and joined back into it. An integrator keeps
  * device buffers sized for its last batch (allocated again, after a device synchronization, when n, y or P change);
  * calls on one integrator must not overlap. */
 enum cb_integrator_flags {
