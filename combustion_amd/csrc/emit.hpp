@@ -40,6 +40,9 @@ struct StraightOptions {
   int32_t group_table = -1;
   unsigned group_size = 1, group_stride = 0;
   std::vector<int32_t> group_offset, increment_offset, increment_state;
+  // With skip_table >= 0 (groups of one block each): in[skip_table] holds one int32 per group, and a block whose
+  // group's value is nonzero returns before any work, writing nothing.
+  int32_t skip_table = -1;
   // Output o of state id is stored at out[o][id · output_stride] (with out[o] = base + o and the stride the number of
   // outputs, state by state).
   unsigned output_stride = 1;

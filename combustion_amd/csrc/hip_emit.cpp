@@ -77,6 +77,8 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
   auto option = [&](const std::vector<int32_t>& v, int32_t r) { return r < (int32_t)v.size() ? v[r] : -1; };
   if (grouped && (o.group_size == 0 || o.block_size % o.group_size != 0 || o.group_stride == 0))
     throw ast::Error("emit: the group size must divide the block size, and groups need values");
+  if (o.skip_table >= 0 && (!grouped || o.group_size != o.block_size))
+    throw ast::Error("emit: skipping groups needs groups of one block each");
   for (int32_t r = 0; r < l.n_input; r++)
     if (option(o.group_offset, r) >= (int32_t)o.group_stride || option(o.increment_offset, r) >= (int32_t)o.group_stride ||
         ((option(o.group_offset, r) >= 0 || option(o.increment_offset, r) >= 0) && !grouped) ||
@@ -106,8 +108,10 @@ std::string emit_straight(const ast::Linear& l, const std::string& name, const S
     const unsigned ng = o.group_stride;
     const unsigned per_block = o.block_size / o.group_size;
     s << "  __shared__ double group_values[" << per_block * ng << "];\n"
-      << "  const unsigned first_group = (offset + blockIdx.x * blockDim.x) / " << o.group_size << "u;\n"
-      << "  const unsigned staged_values = min(" << per_block << "u, (n + " << o.group_size - 1 << "u) / " << o.group_size
+      << "  const unsigned first_group = (offset + blockIdx.x * blockDim.x) / " << o.group_size << "u;\n";
+    if (o.skip_table >= 0)  // the whole block leaves together, before its barrier
+      s << "  if (((const int*)in[" << o.skip_table << "])[first_group] != 0) return;\n";
+    s << "  const unsigned staged_values = min(" << per_block << "u, (n + " << o.group_size - 1 << "u) / " << o.group_size
       << "u - first_group) * " << ng << "u;\n"
       << "  const double* __restrict__ block_groups = in[" << o.group_table << "] + (size_t)first_group * " << ng << "u;\n"
       << "  for (unsigned k = threadIdx.x; k < staged_values; k += blockDim.x) group_values[k] = block_groups[k];\n"

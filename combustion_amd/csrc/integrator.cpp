@@ -57,9 +57,11 @@ StraightOptions shared_options(const ast::Linear& shared, int m) {
 }
 
 // The part that varies with the mass fractions, over the copies (a block of 64 holds whole reactors, whose values it
-// stages in LDS): every input from the reactor's values, copy j = 1..M-1 adding inc_j to component j (Y_{j-1}).
+// stages in LDS): every input from the reactor's values, copy j = 1..M-1 adding inc_j to component j (Y_{j-1}). When
+// a block holds one reactor's copies, it skips a reactor that is not running (in[1]: the reactors' status).
 StraightOptions varying_options(const ast::Linear& varying, int m) {
   StraightOptions o = rhs_options(varying);
+  if (copies_per_reactor(m) == o.block_size) o.skip_table = 1;
   const int32_t B = varying.n_input - (m + 1);
   o.group_table = 0;
   o.group_size = copies_per_reactor(m);
@@ -176,10 +178,11 @@ void Integrator::release() {
   n_ = 0;
   last_y_ = nullptr;
   last_P_ = nullptr;
+  last_status_ = nullptr;
 }
 
-void Integrator::prepare(double* y, const double* P, size_t n) {
-  if (n == n_ && y == last_y_ && P == last_P_) return;
+void Integrator::prepare(double* y, const double* P, const int* status, size_t n) {
+  if (n == n_ && y == last_y_ && P == last_P_ && status == last_status_) return;
   device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   release();
   if (!side_) {
@@ -245,7 +248,7 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
       for (size_t b = 0; b < B; b++) shared_out.push_back(groups_ + off * group_stride(B, m_) + b);
       q.tab_shared_in = table({y + off, P + off});
       q.tab_shared_out = table(shared_out);
-      q.tab_rep_in = table({groups_ + off * group_stride(B, m_)});
+      q.tab_rep_in = table({groups_ + off * group_stride(B, m_), status + off});  // values, and the skip flags
       q.tab_rep_out = table(rows(rep_out, m, nr, off * C));
       std::vector<const void*> t_in = inputs(y);
       t_in[0] = raised_T_ + off;
@@ -257,13 +260,14 @@ void Integrator::prepare(double* y, const double* P, size_t n) {
   n_ = n;
   last_y_ = y;
   last_P_ = P;
+  last_status_ = status;
 }
 
 long Integrator::advance(double* y, const double* P, double* t, double* h, int* status, unsigned* steps, size_t n, double t_end,
                          int max_steps, hipStream_t stream) {
   if (n == 0) return 0;
   if (n >= (size_t(1) << 31)) throw ArgumentError("at most 2^31 - 1 reactors");
-  prepare(y, P, n);
+  prepare(y, P, status, n);
   hipFunction_t running = nullptr, clear = nullptr, copies_kernel = nullptr, stage = nullptr;
   device::check(hipModuleGetFunction(&stage, step_.module, "rosenbrock_stage"), "hipModuleGetFunction");
   device::check(hipModuleGetFunction(&running, step_.module, "rosenbrock_running"), "hipModuleGetFunction");

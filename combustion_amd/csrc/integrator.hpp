@@ -60,7 +60,7 @@ class Integrator {
   // The batch runs in two parts, the first on the caller's stream and the second on side_, so that one part's
   // latency-bound right-hand sides overlap the other's Rosenbrock phases. Each part's pointer tables start at its
   // first reactor: the programs' inputs from y or ys and outputs to F or jac; for difference quotients, the shared
-  // part's inputs [T, P] and outputs, the copies' inputs (the reactors' values) and outputs, and the
+  // part's inputs [T, P] and outputs, the copies' inputs (the reactors' values and status) and outputs, and the
   // temperature-raised states' inputs and outputs.
   struct Part {
     size_t off = 0, n = 0;
@@ -73,8 +73,9 @@ class Integrator {
   hipEvent_t fork_ = nullptr, join_ = nullptr;
   const double* last_y_ = nullptr;
   const double* last_P_ = nullptr;
+  const int* last_status_ = nullptr;
   void release();
-  void prepare(double* y, const double* P, size_t n);
+  void prepare(double* y, const double* P, const int* status, size_t n);
 };
 
 }  // namespace combustion

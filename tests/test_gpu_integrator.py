@@ -162,3 +162,36 @@ def test_a_reactor_does_not_depend_on_its_batch(gpu, mechanism, jacobian, t_end,
     rev, rev_steps = integrate(gpu, M, np.tile(y0[:, order], 3), np.tile(P[order], 3), t_end, jacobian=jacobian, method=method)
     np.testing.assert_array_equal(rev[:, :len(P)], y[:, order])
     np.testing.assert_array_equal(rev_steps[:len(P)], steps[order])
+
+
+@pytest.mark.gpu
+def test_integrator_keeps_the_order_of_the_callers_stream(gpu):
+    """advance() on a stream of the caller's (not the null stream): the states that the stream's earlier work (a few
+    milliseconds of matrix products, then the copy of the initial states) leaves are the ones both halves of the batch
+    start from, and the result equals the null-stream run to the bit."""
+    import torch
+    d = np.load(os.path.join(GOLDEN, "reactor_handwritten.npz"))
+    M = Mechanism(os.path.join(GOLDEN, "handwritten.yaml"))
+    y0, P0 = np.tile(np.vstack([d["T0"], d["Y0"]]), 3), np.tile(d["P"], 3)  # 36 reactors: parts of 24 and 12
+    ref, ref_steps = integrate(gpu, M, y0, P0, 1e-3)
+    n = len(P0)
+    s = torch.cuda.Stream()
+    src = torch.from_numpy(np.ascontiguousarray(y0)).to(gpu)
+    P = torch.from_numpy(np.ascontiguousarray(P0)).to(gpu)
+    a = torch.randn(4096, 4096, device=gpu)
+    y = torch.zeros_like(src)
+    t = torch.zeros(n, dtype=torch.float64, device=gpu)
+    h = torch.full((n,), 1e-9, dtype=torch.float64, device=gpu)
+    status = torch.zeros(n, dtype=torch.int32, device=gpu)
+    steps = torch.zeros(n, dtype=torch.int32, device=gpu)
+    integ = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12)
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(8):
+            a = a @ a / 64.0
+        y.copy_(src)
+        integ.advance(y, P, t, h, status, steps, 1e-3, max_steps=200000, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.all(status == 1)
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    np.testing.assert_array_equal(steps.cpu().numpy(), ref_steps)
