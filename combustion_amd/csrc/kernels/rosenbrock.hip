@@ -139,7 +139,6 @@ __device__ __forceinline__ constexpr double rodas_c(int i, int j) {
                 : -6.058818238834054;
 }
 #define GAMMA 0.25
-#define LAST_PHASE 6
 #define ERROR_ORDER 4.0  // the embedded estimate's local error is O(h^4)
 #define FACTOR_MAX 6.0
 
@@ -168,7 +167,6 @@ __device__ __forceinline__ double rodas_stage(const double (&a)[M], int pos, con
 }
 #else
 #define GAMMA D_ROS
-#define LAST_PHASE 3
 #define ERROR_ORDER 3.0  // the error estimate's local error is O(h^3)
 #define FACTOR_MAX 5.0
 #endif
