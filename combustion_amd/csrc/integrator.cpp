@@ -168,6 +168,9 @@ Integrator::~Integrator() {
   release();
   if (fork_) (void)hipEventDestroy(fork_);
   if (join_) (void)hipEventDestroy(join_);
+  for (hipEvent_t e : polled_)
+    if (e) (void)hipEventDestroy(e);
+  if (host_count_) (void)hipHostFree(host_count_);
   if (side_) (void)hipStreamDestroy(side_);
 }
 
@@ -189,6 +192,8 @@ void Integrator::prepare(double* y, const double* P, const int* status, size_t n
     device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
     device::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
     device::check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreateWithFlags");
+    for (hipEvent_t& e : polled_) device::check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+    device::check(hipHostMalloc((void**)&host_count_, 2 * sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc");
   }
   const size_t m = (size_t)m_;
   auto alloc = [&](size_t bytes) {
@@ -204,7 +209,7 @@ void Integrator::prepare(double* y, const double* P, const int* status, size_t n
   lu_ = (double*)alloc(m * m * n * sizeof(double));
   rd_ = (double*)alloc(m * n * sizeof(double));
   perm_ = (int*)alloc(m * n * sizeof(int));
-  running_ = (unsigned*)alloc(256);
+  running_ = (unsigned*)alloc(256);  // the running counts of the last two polls
   const size_t B = differences_ ? shared_linear_.output.size() : 0, C = copies_per_reactor(m_), nr = n * C;
   double *rep_out = nullptr, *t_out = nullptr;
   if (differences_) {
@@ -282,7 +287,8 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   // the second part's stream starts after the work already on the caller's stream
   device::check(hipEventRecord(fork_, stream), "hipEventRecord");
   device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
-  long taken = 0;
+  long taken = 0, polled_steps[2] = {0, 0};
+  int polls = 0;
   for (int step = 0; step < max_steps; step++) {
     for (size_t p = 0; p < parts_.size(); p++) {
       const Part& q = parts_[p];
@@ -318,19 +324,34 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
     }
     taken++;
     if ((step + 1) % kPollEvery == 0 || step + 1 == max_steps) {
-      // the count waits for both parts' steps so far
+      // the count waits for both parts' steps so far; it goes to pinned memory behind an event, and the host reads
+      // the previous poll's count instead, which the GPU has usually passed: the GPU never idles for the round trip.
+      // Reactors that are not running are left alone, so the steps launched after a poll that found none running
+      // change nothing, and the call reports the steps up to that poll.
+      const int slot = polls & 1;
       device::check(hipEventRecord(join_, side_), "hipEventRecord");
       device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
-      void* cargs[] = {&running_};
+      unsigned* count = running_ + slot;
+      void* cargs[] = {&count};
       device::check(hipModuleLaunchKernel(clear, 1, 1, 1, 64, 1, 1, 0, stream, cargs, nullptr), "launch rosenbrock_clear");
-      void* rargs[] = {&n32, &status, &running_};
+      void* rargs[] = {&n32, &status, &count};
       device::check(hipModuleLaunchKernel(running, (n32 + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr), "launch rosenbrock_running");
-      unsigned left = 0;
-      device::check(hipMemcpyAsync(&left, running_, sizeof left, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
-      device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      if (left == 0) break;
+      device::check(hipMemcpyAsync(host_count_ + slot, count, sizeof(unsigned), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+      device::check(hipEventRecord(polled_[slot], stream), "hipEventRecord");
+      polled_steps[slot] = taken;
+      if (++polls >= 2) {
+        device::check(hipEventSynchronize(polled_[slot ^ 1]), "hipEventSynchronize");
+        if (host_count_[slot ^ 1] == 0) {
+          taken = polled_steps[slot ^ 1];
+          break;
+        }
+      }
     }
   }
+  // every step launched is complete on return
+  device::check(hipEventRecord(join_, side_), "hipEventRecord");
+  device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
+  device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   return taken;
 }
 

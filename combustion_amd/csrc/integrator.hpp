@@ -71,6 +71,8 @@ class Integrator {
   std::vector<Part> parts_;
   hipStream_t side_ = nullptr;
   hipEvent_t fork_ = nullptr, join_ = nullptr;
+  hipEvent_t polled_[2] = {nullptr, nullptr};  // the last two polls' counts copied to host_count_ (pinned)
+  unsigned* host_count_ = nullptr;
   const double* last_y_ = nullptr;
   const double* last_P_ = nullptr;
   const int* last_status_ = nullptr;
