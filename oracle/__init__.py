@@ -1,7 +1,7 @@
 """ORACLE — test infrastructure only.
 
-CPU restatement of the reference's per-state chemistry, used as the checker of the HIP path (and timed, for
-information, beside the CPU baseline, which is the product's own host-compiled program). Only tests/,
+CPU restatement of the reference's per-state chemistry, against which the tests compare the HIP path (and timed,
+for information, beside the CPU baseline, which is the product's own host-compiled program). Only tests/,
 __graft_entry__.smoke() and bench.py (its error samples and cpu_baseline leg) may import it; the product
 (combustion_amd) never does.
 

@@ -1,5 +1,5 @@
 /* ORACLE — test infrastructure only. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * may load this library, and only as the checker or the CPU baseline, never as the product path.
+ * may load this library, and only to check results or to time a CPU baseline, never as the product path.
  *
  * Closed-form FP64 restatement of the per-state chemistry, written from textbook formulas rather than from
  * the reference's expression rearrangements, so that it pins them:
