@@ -197,10 +197,11 @@ void cb_cpu_module_free(cb_cpu_module* mod);
  * cb_integrator_advance moves n reactors toward t_end, in place, every array in device memory: y [K+1][n] (SoA rows
  * T, Y_0..Y_{K-1}), P [n] (constant), t [n] and h [n] (each reactor's time and next step size), status [n] (0
  * running; set to 1 at t_end, 2 when the step size collapsed; others are left alone), steps [n] (attempted steps).
- * It stops after max_steps steps of the batch or when no reactor runs, and blocks until then. Half of the batch runs
- * on a stream of the integrator's own, after the work already on `stream` and joined back into it. An integrator
- * keeps device buffers sized for its last batch (allocated again, after a device synchronization, when n, y, P or
- * status change); calls on one integrator must not overlap. */
+ * It stops after max_steps steps of the batch or when no reactor runs (found by a poll every 8 steps; the steps
+ * launched before the host reads that poll leave every reactor alone and are not counted in *taken), and blocks
+ * until then. Half of the batch runs on a stream of the integrator's own, after the work already on `stream` and
+ * joined back into it. An integrator keeps device buffers sized for its last batch (allocated again, after a device
+ * synchronization, when n, y, P or status change); calls on one integrator must not overlap. */
 enum cb_integrator_flags {
   CB_JACOBIAN_AUTOMATIC = 0,   /* the analytic program up to 8 state rows (K <= 7), difference quotients above */
   CB_JACOBIAN_ANALYTIC = 1,    /* cb_build_jacobian's forward-mode program, f and J in one straight-line kernel */
