@@ -3,8 +3,8 @@
 // fractions — its temperature and pressure part once per reactor, the rest over the reactor's K+1 copies, each copy
 // raising one component — and the whole right-hand side at the raised temperature, whose outputs the first
 // Rosenbrock phase turns into J), the Rosenbrock phases of kernels/rosenbrock.hip (three for ode23s, six for
-// RODAS4) with a right-hand-side evaluation between two phases; the batch in two parts, on the caller's stream and
-// on a stream of the integrator's own, polled for reactors still running every few steps.
+// RODAS4) with a right-hand-side evaluation between two phases; the batch in parts, on the caller's stream and on
+// streams of the integrator's own, polled for reactors still running every few steps.
 #include "integrator.hpp"
 
 #include <algorithm>
@@ -143,6 +143,9 @@ Integrator::Integrator(std::shared_ptr<const Mechanism> m, double rtol, double a
   IntegratorPrograms p(*mech_, flags);
   differences_ = p.differences;
   phases_ = (flags & kMethodOde23s) ? 3 : 6;
+  // one reactor per wave (state size above 32): each launch has work enough for the host to keep four streams fed;
+  // the packed waves' launches are short, and two parts keep the launches per step within the host's rate
+  parts_count_ = m_ > 32 ? 4 : 2;
   systems_per_wave_ = (flags & kOneSystemPerWave) ? 1u : 64u / (unsigned)m_;
   rhs_function_ = std::move(p.rhs_function);
   rhs_linear_ = std::move(p.rhs_linear);
@@ -171,7 +174,7 @@ Integrator::~Integrator() {
   for (hipEvent_t e : polled_)
     if (e) (void)hipEventDestroy(e);
   if (host_count_) (void)hipHostFree(host_count_);
-  if (side_) (void)hipStreamDestroy(side_);
+  for (hipStream_t s : sides_) (void)hipStreamDestroy(s);
 }
 
 void Integrator::release() {
@@ -188,8 +191,9 @@ void Integrator::prepare(double* y, const double* P, const int* status, size_t n
   if (n == n_ && y == last_y_ && P == last_P_ && status == last_status_) return;
   device::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   release();
-  if (!side_) {
-    device::check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  if (sides_.empty()) {
+    sides_.resize(parts_count_ - 1);
+    for (hipStream_t& s : sides_) device::check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
     device::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
     device::check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreateWithFlags");
     for (hipEvent_t& e : polled_) device::check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
@@ -220,17 +224,20 @@ void Integrator::prepare(double* y, const double* P, const int* status, size_t n
     t_out = (double*)alloc(m * n * sizeof(double));
     raised_T_ = (double*)alloc(n * sizeof(double));
   }
-  // the batch in two parts, each on its own stream; the first part's length a multiple of 8 (64-byte rows)
-  const size_t first = std::min(n, ((n + 1) / 2 + 7) & ~(size_t)7);
+  // the batch in parts_count_ parts, each on its own stream, each starting at a multiple of 8 reactors (64-byte rows)
+  const int parts = parts_count_;
+  std::vector<size_t> bounds;
+  for (int k = 0; k <= parts; k++) bounds.push_back(k == parts ? n : std::min(n, (n * k / parts + 7) & ~(size_t)7));
   auto table = [&](const std::vector<const void*>& ptrs) {
     void** t = (void**)alloc(ptrs.size() * sizeof(void*));
     device::check(hipMemcpy(t, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice), "hipMemcpy table");
     return t;
   };
-  for (size_t off : {size_t(0), first}) {
+  for (int k = 0; k < parts; k++) {
+    const size_t off = bounds[k];
     Part q;
     q.off = off;
-    q.n = off == 0 ? first : n - first;
+    q.n = bounds[k + 1] - off;
     if (q.n == 0) continue;
     // pointer tables of the programs: inputs [T, P, Y_0..Y_{K-1}] from y or ys; outputs F rows or jac rows; all
     // starting at the part's first reactor
@@ -268,6 +275,15 @@ void Integrator::prepare(double* y, const double* P, const int* status, size_t n
   last_status_ = status;
 }
 
+// The caller's stream waits for the work enqueued so far on the other parts' streams (one event, recorded again for
+// each: a wait takes the event's last record at the time it is enqueued).
+void Integrator::join_sides(hipStream_t stream) {
+  for (hipStream_t s : sides_) {
+    device::check(hipEventRecord(join_, s), "hipEventRecord");
+    device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
+  }
+}
+
 long Integrator::advance(double* y, const double* P, double* t, double* h, int* status, unsigned* steps, size_t n, double t_end,
                          int max_steps, hipStream_t stream) {
   if (n == 0) return 0;
@@ -284,15 +300,15 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
   double te = t_end, rtol = rtol_, atol0 = atol_T_, atol = atol_Y_;
   const size_t m = (size_t)m_, C = copies_per_reactor(m_);
   unsigned stride = (unsigned)group_stride(shared_linear_.output.size(), m_), shared_values = (unsigned)shared_linear_.output.size();
-  // the second part's stream starts after the work already on the caller's stream
+  // the other parts' streams start after the work already on the caller's stream
   device::check(hipEventRecord(fork_, stream), "hipEventRecord");
-  device::check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
+  for (hipStream_t s : sides_) device::check(hipStreamWaitEvent(s, fork_, 0), "hipStreamWaitEvent");
   long taken = 0, polled_steps[2] = {0, 0};
   int polls = 0;
   for (int step = 0; step < max_steps; step++) {
     for (size_t p = 0; p < parts_.size(); p++) {
       const Part& q = parts_[p];
-      hipStream_t st = p == 0 ? stream : side_;
+      hipStream_t st = p == 0 ? stream : sides_[p - 1];
       const size_t off = q.off, count = q.n;
       unsigned c32 = (unsigned)count;
       if (differences_) {
@@ -324,18 +340,20 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
     }
     taken++;
     if ((step + 1) % kPollEvery == 0 || step + 1 == max_steps) {
-      // the count waits for both parts' steps so far; it goes to pinned memory behind an event, and the host reads
+      // the count waits for every part's steps so far; it goes to pinned memory behind an event, and the host reads
       // the previous poll's count instead, which the GPU has usually passed: the GPU never idles for the round trip.
       // Reactors that are not running are left alone, so the steps launched after a poll that found none running
       // change nothing, and the call reports the steps up to that poll.
       const int slot = polls & 1;
-      device::check(hipEventRecord(join_, side_), "hipEventRecord");
-      device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
+      join_sides(stream);
       unsigned* count = running_ + slot;
       void* cargs[] = {&count};
       device::check(hipModuleLaunchKernel(clear, 1, 1, 1, 64, 1, 1, 0, stream, cargs, nullptr), "launch rosenbrock_clear");
       void* rargs[] = {&n32, &status, &count};
       device::check(hipModuleLaunchKernel(running, (n32 + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr), "launch rosenbrock_running");
+      // the other parts go on after the count only, so that it sees every part at this step, not later ones
+      device::check(hipEventRecord(fork_, stream), "hipEventRecord");
+      for (hipStream_t s : sides_) device::check(hipStreamWaitEvent(s, fork_, 0), "hipStreamWaitEvent");
       device::check(hipMemcpyAsync(host_count_ + slot, count, sizeof(unsigned), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
       device::check(hipEventRecord(polled_[slot], stream), "hipEventRecord");
       polled_steps[slot] = taken;
@@ -349,8 +367,7 @@ long Integrator::advance(double* y, const double* P, double* t, double* h, int* 
     }
   }
   // every step launched is complete on return
-  device::check(hipEventRecord(join_, side_), "hipEventRecord");
-  device::check(hipStreamWaitEvent(stream, join_, 0), "hipStreamWaitEvent");
+  join_sides(stream);
   device::check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   return taken;
 }

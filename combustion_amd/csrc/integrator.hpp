@@ -46,6 +46,7 @@ class Integrator {
   double rtol_, atol_T_, atol_Y_;
   bool differences_;
   int phases_;  // Rosenbrock phases per step: 3 (ode23s) or 6 (RODAS4)
+  int parts_count_;  // the batch's parts, each on its own stream
   unsigned systems_per_wave_;
   ast::Function rhs_function_, jacobian_function_, shared_function_, varying_function_;
   ast::Linear rhs_linear_, jacobian_linear_, shared_linear_, varying_linear_;
@@ -57,8 +58,8 @@ class Integrator {
   int* perm_ = nullptr;
   unsigned* running_ = nullptr;
   double *groups_ = nullptr, *raised_T_ = nullptr;  // each reactor's values for its copies; the raised temperatures
-  // The batch runs in two parts, the first on the caller's stream and the second on side_, so that one part's
-  // latency-bound right-hand sides overlap the other's Rosenbrock phases. Each part's pointer tables start at its
+  // The batch runs in parts_count_ parts, the first on the caller's stream and the others on sides_, so that one
+  // part's latency-bound right-hand sides overlap another's Rosenbrock phases. Each part's pointer tables start at its
   // first reactor: the programs' inputs from y or ys and outputs to F or jac; for difference quotients, the shared
   // part's inputs [T, P] and outputs, the copies' inputs (the reactors' values and status) and outputs, and the
   // temperature-raised states' inputs and outputs.
@@ -69,7 +70,7 @@ class Integrator {
     void **tab_t_in = nullptr, **tab_t_out = nullptr;
   };
   std::vector<Part> parts_;
-  hipStream_t side_ = nullptr;
+  std::vector<hipStream_t> sides_;  // the streams of the parts after the first
   hipEvent_t fork_ = nullptr, join_ = nullptr;
   hipEvent_t polled_[2] = {nullptr, nullptr};  // the last two polls' counts copied to host_count_ (pinned)
   unsigned* host_count_ = nullptr;
@@ -78,6 +79,7 @@ class Integrator {
   const int* last_status_ = nullptr;
   void release();
   void prepare(double* y, const double* P, const int* status, size_t n);
+  void join_sides(hipStream_t stream);
 };
 
 }  // namespace combustion

@@ -199,7 +199,7 @@ void cb_cpu_module_free(cb_cpu_module* mod);
  * running; set to 1 at t_end, 2 when the step size collapsed; others are left alone), steps [n] (attempted steps).
  * It stops after max_steps steps of the batch or when no reactor runs (found by a poll every 8 steps; the steps
  * launched before the host reads that poll leave every reactor alone and are not counted in *taken), and blocks
- * until then. Half of the batch runs on a stream of the integrator's own, after the work already on `stream` and
+ * until then. Parts of the batch run on streams of the integrator's own, after the work already on `stream` and
  * joined back into it. An integrator keeps device buffers sized for its last batch (allocated again, after a device
  * synchronization, when n, y, P or status change); calls on one integrator must not overlap. */
 enum cb_integrator_flags {

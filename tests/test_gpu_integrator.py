@@ -129,10 +129,11 @@ def integrate(gpu, M, y0, P, t_end, **flags):
     h = torch.full((n,), 1e-9, dtype=torch.float64, device=gpu)
     status = torch.zeros(n, dtype=torch.int32, device=gpu)
     steps = torch.zeros(n, dtype=torch.int32, device=gpu)
-    Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, **flags).advance(y, torch.from_numpy(np.ascontiguousarray(P)).to(gpu), t, h,
-                                                                         status, steps, t_end, max_steps=200000)
+    taken = Integrator(M, rtol=1e-8, atol_T=1e-6, atol_Y=1e-12, **flags).advance(y, torch.from_numpy(np.ascontiguousarray(P)).to(gpu), t,
+                                                                                 h, status, steps, t_end, max_steps=200000)
     torch.cuda.synchronize()
     assert torch.all(status == 1)
+    assert taken >= int(steps.max().item())  # the batch's steps cover every reactor's, whichever stream it ran on
     return y.cpu().numpy(), steps.cpu().numpy()
 
 
