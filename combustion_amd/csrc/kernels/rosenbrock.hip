@@ -11,8 +11,8 @@
 //   phase 1 (after [f(y), J(y)]):  W factored; k1 = W⁻¹ γ h f(y); ys = Y_2 = y + a_21 k1
 //   phase I = 2..6 (after f(Y_I)): k_I = W⁻¹ γ (h f(Y_I) + Σ_{j<I} c_Ij k_j); ys = Y_{I+1} = y + Σ_{j<=I} a_{I+1,j} k_j
 //                                   (I < 6); error = k6, y_new = Y_6 + k6
-// The last phase accepts (y = y_new, t += h) when max_i |error_i| / (atol_i + rtol max(|y_i|, |y_new,i|)) <= 1 and
-// takes the next h from that ratio (0.9 ratio^(-1/q), q the order of the error estimate's local error, within
+// The last phase accepts (y = y_new, t += h) when the RMS over i of error_i / (atol_i + rtol max(|y_i|, |y_new,i|))
+// (CVODE's weighted RMS norm) is at most 1, and takes the next h from that ratio (0.9 ratio^(-1/q), q the order of the error estimate's local error, within
 // [0.2, 5] for ode23s and [0.2, 6] for RODAS4, at most 0.5 after a rejection), clipped to the end time.
 // A wave holds P = ⌊64/M⌋ systems, M lanes each: lane M p + i holds row i of
 // system p's W (M doubles in registers). With one system per wave the pivot row of each elimination step is
@@ -105,19 +105,12 @@ __device__ __forceinline__ double increment(double yj, int j, double rtol, doubl
   return 1.4901161193847656e-08 * fmax(fabs(yj), (j == 0 ? atol0 : atol) / rtol);
 }
 
-// The largest value over this lane's system.
-__device__ __forceinline__ double system_max(double v, int base) {
-#if P == 1
-  (void)base;
+// The sum of a value over this lane's system, added in row order whatever the packing (every lane takes part).
+__device__ __forceinline__ double system_sum(double v, int base) {
+  double sum = 0.0;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-#else
-  double m = v;
-#pragma unroll
-  for (int r = 0; r < M; r++) m = fmax(m, __shfl(v, base + r));
-  return m;
-#endif
+  for (int r = 0; r < M; r++) sum += row_value(v, r, base);
+  return sum;
 }
 
 #ifdef RODAS
@@ -355,11 +348,13 @@ __device__ __forceinline__ void rosenbrock_phase(int phase, unsigned n, size_t l
   const double y0 = row ? y[(size_t)ir * ld + sv] : 0.0, y1 = row ? ys[(size_t)ir * ld + sv] : 0.0;
   const double err = hs * (x1 - 2.0 * x2 + x3) * (1.0 / 6.0);
 #endif
-  // step control: accept when the scaled error is at most 1; the next h from it
+  // step control: accept when the scaled error's root mean square over the components is at most 1 (the weighted
+  // RMS norm of CVODE); the next h from it
   const double scale = (i == 0 ? atol0 : atol) + rtol * fmax(fabs(y0), fabs(y1));
-  double ratio = row ? fabs(err) / scale : 0.0;
-  if (!(ratio == ratio)) ratio = 1e300;  // a NaN (a singular W, say) rejects the step
-  ratio = system_max(ratio, base);
+  double q = row ? err / scale : 0.0;
+  q *= q;
+  if (!(q == q)) q = 1e300;  // a NaN (a singular W, say) rejects the step
+  const double ratio = sqrt(system_sum(q, base) * (1.0 / M));
   if (!active) return;
   const bool accept = ratio <= 1.0;
   const double factor = fmin(FACTOR_MAX, fmax(0.2, 0.9 * pow(fmax(ratio, 1e-12), -1.0 / ERROR_ORDER)));

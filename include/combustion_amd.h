@@ -190,10 +190,10 @@ void cb_cpu_module_free(cb_cpu_module* mod);
  * wrapper (cvode/lib.rs:17-47: CVodeCreate(CV_BDF), CVodeSStolerances(rtol, atol), dense linear solver, one step at
  * a time) and of the reactor loop of plot.rs:80-118, for many reactors at once: the stiffly accurate order-4
  * Rosenbrock method RODAS4 (Hairer & Wanner, with its embedded order-3 estimate; or, with CB_INTEGRATOR_ODE23S, the
- * L-stable (2,3) pair of ode23s, Shampine & Reichelt 1997) with the Jacobian of cb_build_reactor's program (analytic,
- * cb_build_jacobian, or difference quotients: flags) and floor(64/(K+1)) reactors per wave for the LU factorisation
- * and the solves (state size K+1 <= 64). Tolerances: rtol, and
- * atol_T (temperature) / atol_Y (mass fractions). With out == NULL the kernels are only compiled into the cache.
+ * L-stable (2,3) pair of ode23s, Shampine & Reichelt 1997; the error test in CVODE's weighted RMS norm) with the
+ * Jacobian of cb_build_reactor's program (analytic, cb_build_jacobian, or difference quotients: flags) and
+ * floor(64/(K+1)) reactors per wave for the LU factorisation and the solves (state size K+1 <= 64). Tolerances: rtol,
+ * and atol_T (temperature) / atol_Y (mass fractions). With out == NULL the kernels are only compiled into the cache.
  * cb_integrator_advance moves n reactors toward t_end, in place, every array in device memory: y [K+1][n] (SoA rows
  * T, Y_0..Y_{K-1}), P [n] (constant), t [n] and h [n] (each reactor's time and next step size), status [n] (0
  * running; set to 1 at t_end, 2 when the step size collapsed; others are left alone), steps [n] (attempted steps).

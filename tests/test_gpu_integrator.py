@@ -4,9 +4,9 @@ wave for the LU and the solves), the counterpart of the reference's CVODE wrappe
 against the committed scipy BDF trajectories (tests/golden/make_reactor_golden.py, rtol 1e-10 on the FP64 oracle's
 right-hand side) at 1, 3 and 10 ms, with the analytic Jacobian and with difference quotients, several reactors per
 wave and one. Bars: with the integrator's tolerances at rtol 1e-8, atol 1e-6 K and 1e-12, temperatures within 2e-6
-relative and mass fractions within 1e-6 absolute of the reference trajectory for ode23s (measured: 1.6e-7 and 8.5e-8
+relative and mass fractions within 1e-6 absolute of the reference trajectory for ode23s (measured: 2.9e-7 and 1.6e-7
 at 1 ms, mid-ignition for some reactors, where timing differences weigh most), within 1e-8 for RODAS4 (measured:
-4.3e-10 and 5.0e-10)."""
+3.6e-10 and 1.3e-9)."""
 import os
 
 import numpy as np
@@ -61,8 +61,8 @@ def test_reactors_through_ignition_match_scipy_bdf(gpu, jacobian, one_per_wave, 
 def test_large_mechanism_reactors_match_scipy_bdf(gpu, method):
     """The 53-species mechanism (state size 54: difference quotients, one reactor per wave) against the scipy BDF
     trajectories of tests/golden/reactor_gri53.npz (four fuel/O2/N2 mixtures to 1 ms), at ode23s's bars for both
-    methods (measured at 0.1 ms: ΔT/T 4.8e-7 for ode23s, 4.4e-7 for RODAS4, ΔY 9.0e-8 for both: the fixture's own
-    error on one reactor: a CPU run of RODAS4 at rtol 1e-11 differs from it by the same 4.4e-7 and 9.0e-8)."""
+    methods (measured at 0.1 ms: ΔT/T 4.9e-7 for ode23s, 4.8e-7 for RODAS4, ΔY 9.0e-8 for both: the fixture's own
+    error on one reactor: a CPU run of RODAS4 at rtol 1e-11 differs from it by 4.4e-7 and 9.0e-8)."""
     from combustion_amd import synth
     d = np.load(os.path.join(GOLDEN, "reactor_gri53.npz"))
     M = Mechanism(synth.mechanism_path("gri-53"))
@@ -203,8 +203,8 @@ def test_mid_size_mechanism_packings_and_methods_agree(gpu):
     """The 10-species mechanism (state size 11: five reactors per wave; 16 copies per reactor for the difference
     quotients, four reactors to a block of copies), which no trajectory fixture covers: eight fuel/O2/N2 reactors
     (no radicals at the start) to 1 µs with RODAS4 on packed waves, with one reactor per wave, and with ode23s. The
-    packings agree to the bit (the same arithmetic in either); at rtol 1e-8 the methods agree within 1e-6 (relative
-    T, absolute Y; measured 1.2e-7 and 3.3e-8, CPU runs of the two methods on the same reactors 1.4e-7 and 3.8e-8).
+    packings agree to the bit (the same arithmetic in either, the error norm summed in row order); at rtol 1e-8 the
+    methods agree within 1e-6 (relative T, absolute Y; measured 3.4e-7 and 8.9e-8).
     (With the analytic Jacobian these reactors stall at steps below 1e-19 s: the mechanism's radical rates are so
     nonlinear at zero concentration that the exact Jacobian at the state misses the stiffness a trace of radicals
     brings, which the difference quotients' increments reach into; the automatic choice takes those above 8 rows.)"""
